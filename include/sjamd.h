@@ -1,0 +1,192 @@
+/*
+ * sjamd.h -- C ABI of the MI355X (gfx950) grid-indexed epsilon self-join.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes on every
+ * call, no exits.  Device pointers are HIP device addresses; `stream` is a
+ * hipStream_t passed as void* (NULL: the context creates its own).
+ *
+ * Each entry point names the reference (benoitgallet/self-join-hpbdc2019)
+ * interface it replaces.
+ */
+#ifndef SJAMD_H
+#define SJAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SJ_API __attribute__((visibility("default")))
+#define SJ_MAXK 12 /* indexed dimensions supported by the 3^k stencil */
+#define SJ_MAXD 64 /* point dimensions */
+
+typedef enum sj_status {
+    SJ_OK = 0,
+    SJ_ERR_INVALID = 1,     /* bad argument (eps <= 0, NaN coordinate, k > d ...) */
+    SJ_ERR_NOMEM = 2,       /* device or host allocation failed */
+    SJ_ERR_HIP = 3,         /* HIP runtime error */
+    SJ_ERR_OVERFLOW = 4,    /* cell count / coordinate / pair count overflow */
+    SJ_ERR_UNSUPPORTED = 5, /* configuration not supported */
+    SJ_ERR_IO = 6,          /* file could not be read */
+    SJ_ERR_STATE = 7        /* result not available in the requested view */
+} sj_status;
+
+/* Traversal forms: reference search modes 3 (FULL), 4 (UNICOMP),
+ * 5 (BUNICOMP), 6 and 10 (LID); main.cu:88-150, GPU.cu:2162-2283. */
+typedef enum sj_form {
+    SJ_FORM_FULL = 0,
+    SJ_FORM_UNICOMP = 1,
+    SJ_FORM_BUNICOMP = 2,
+    SJ_FORM_LID = 3
+} sj_form;
+
+typedef enum sj_precision {
+    SJ_PREC_FP64 = 0,        /* reference predicate, kernel.cu:341-349 */
+    SJ_PREC_FP32_RECHECK = 1 /* FP32 bulk + FP64 re-decision of the margin band */
+} sj_precision;
+
+/* Query-point scheduling: strided batches (modes 3-6, kernel.cu:978) or the
+ * workload-sorted atomic work queue (modes 9-10, kernel.cu:2202-2236). */
+typedef enum sj_schedule {
+    SJ_SCHED_STRIDED = 0,
+    SJ_SCHED_WORKQUEUE = 1
+} sj_schedule;
+
+typedef struct sj_context_s *sj_context;
+typedef struct sj_grid_s *sj_grid;
+typedef struct sj_result_s *sj_result;
+
+typedef struct sj_join_params {
+    int32_t form;         /* sj_form */
+    int32_t precision;    /* sj_precision */
+    int32_t schedule;     /* sj_schedule */
+    int32_t sort_rows;    /* 1: each neighbour row ascending by point ID */
+    uint64_t capacity;    /* result pairs per batch; 0 = whole result on device */
+    int32_t block_size;   /* threads per block of the traversal kernels (0: 256) */
+    int32_t queue_chunk;  /* query points per work-queue grab (0: 64) */
+    int32_t cells_per_tile; /* reserved for tiled traversal (0: auto) */
+    int32_t reserved;
+} sj_join_params;
+
+typedef struct sj_grid_info {
+    uint64_t n;
+    int32_t d, k;
+    double eps;
+    double minArr[SJ_MAXK];   /* min - eps per indexed dim */
+    double maxArr[SJ_MAXK];   /* max + eps per indexed dim */
+    uint32_t nCells[SJ_MAXK];
+    uint64_t totalCells;
+    uint64_t nNonEmpty;
+    int32_t dense_lookup;     /* 1: O(1) cell table, 0: binary search */
+    int32_t reserved;
+} sj_grid_info;
+
+typedef struct sj_stats {
+    uint64_t total_pairs;        /* |S|, reference totalNeighbors */
+    uint64_t num_batches;
+    uint64_t candidates;         /* distance evaluations of the count pass */
+    uint64_t fp32_band_pairs;    /* candidates re-decided in FP64 */
+    uint64_t fp32_misclassified; /* pairs whose plain FP32 decision was wrong */
+    uint64_t overflow_retries;
+    double ms_grid;              /* device time, grid construction */
+    double ms_workload;          /* device time, workload estimate + order */
+    double ms_count;             /* device time, count pass */
+    double ms_scan;              /* device time, offsets */
+    double ms_fill;              /* device time, fill pass (all batches) */
+    double ms_sort_rows;         /* device time, canonical row sort */
+    double ms_total;             /* wall time of the call */
+} sj_stats;
+
+/* Row of the reference neighbour table (structs.h:41-47): neighbours of
+ * pointID are ids[indexmin..indexmax] of batch `batch`. */
+typedef struct sj_neighbor_lookup {
+    int32_t pointID;
+    int32_t indexmin;
+    int32_t indexmax;
+    int32_t batch;
+} sj_neighbor_lookup;
+
+/* Last error message of the calling thread. */
+SJ_API const char *sj_last_error(void);
+
+/* Replaces warmUpGPU (GPU.h:54, GPU.cu:2572): selects the device, creates
+ * the stream and the caching device pool. */
+SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out);
+SJ_API void sj_context_destroy(sj_context ctx);
+SJ_API sj_status sj_context_set_stream(sj_context ctx, void *stream);
+
+/* Replaces generateNDGridDimensions + populateNDGridIndexAndLookupArray
+ * (main.cu:30-31,240-503).  d_points: n x d row-major float64 in device
+ * memory; the first k coordinates are indexed.  Built on the device. */
+SJ_API sj_status sj_grid_build(sj_context ctx, const double *d_points, uint64_t n,
+                               int32_t d, int32_t k, double eps, sj_grid *out);
+SJ_API sj_status sj_grid_get_info(sj_grid g, sj_grid_info *info);
+/* Host copies of the grid arrays (main.cu:240-413 outputs): cellLinear =
+ * gridCellLookupArr[].gridLinearID, indexMin/indexMax = index[], indexLookup =
+ * indexLookupArr, cellCoord = per-cell coordinates [nNonEmpty x k].  Any
+ * pointer may be NULL. */
+SJ_API sj_status sj_grid_copy_arrays(sj_grid g, uint64_t *cellLinear, int32_t *indexMin,
+                                     int32_t *indexMax, uint32_t *indexLookup,
+                                     uint32_t *cellCoord);
+/* gridCellNDMask / gridCellNDMaskOffsets / nNDMaskElems (main.cu:381-401).
+ * mask may be NULL to query nElems. */
+SJ_API sj_status sj_grid_nd_mask(sj_grid g, uint32_t *mask, uint32_t *offsets, uint64_t *nElems);
+SJ_API void sj_grid_free(sj_grid g);
+
+/* Replaces distanceTableNDGridBatches (GPU.h:37-40, GPU.cu:1297-2569).
+ * Produces the neighbour table; see the sj_result_* views. */
+SJ_API sj_status sj_join(sj_grid g, const sj_join_params *params, sj_result *out);
+
+/* Grid build + join in one call (the reference's timed region,
+ * main.cu:182-195, with the grid built on the device). */
+SJ_API sj_status sj_self_join(sj_context ctx, const double *d_points, uint64_t n, int32_t d,
+                              int32_t k, double eps, const sj_join_params *params,
+                              sj_result *out);
+
+SJ_API sj_status sj_result_get_stats(sj_result r, sj_stats *stats);
+SJ_API uint64_t sj_result_total_pairs(sj_result r);
+SJ_API uint64_t sj_result_num_points(sj_result r);
+/* Device CSR in original point-ID order: offsets[n+1] (uint64), ids[|S|].
+ * Available when the result is a single device-resident batch. */
+SJ_API sj_status sj_result_device_csr(sj_result r, const uint64_t **d_offsets,
+                                      const int32_t **d_ids);
+/* Host copy of the CSR (any batching). */
+SJ_API sj_status sj_result_copy_csr(sj_result r, uint64_t *offsets, int32_t *ids);
+/* Key/value view, key-sorted: replaces the thrust::sort_by_key output
+ * (GPU.cu:2336-2349).  keys, vals: |S| entries each. */
+SJ_API sj_status sj_result_copy_pairs(sj_result r, int32_t *keys, int32_t *vals);
+/* Reference neighbour table (constructNeighborTableKeyValueWithPtrs,
+ * GPU.h:52, GPU.cu:2595-2637): table[n]; data of batch b via
+ * sj_result_batch_data. */
+SJ_API sj_status sj_result_neighbor_table(sj_result r, sj_neighbor_lookup *table);
+SJ_API sj_status sj_result_batch_data(sj_result r, uint64_t batch, const int32_t **ids,
+                                      uint64_t *count);
+SJ_API void sj_result_free(sj_result r);
+
+/* Workload per non-empty cell (sortByWorkLoad / sortByWorkLoadLidUnicomp,
+ * kernel.cu:33-250) and the descending order (GPU.cu:1695-1698). */
+SJ_API sj_status sj_workload(sj_grid g, int32_t lid, int64_t *nbPoints, uint32_t *order);
+
+/* Batch estimator (callGPUBatchEst / callGPUBatchEstWorkQueue, GPU.h:42-50,
+ * GPU.cu:854-1284): sampled count and the buffer/batch policy. */
+SJ_API sj_status sj_batch_estimate(sj_grid g, uint64_t stride, uint64_t *sampleCount,
+                                   uint64_t *estimated, uint64_t *numBatches,
+                                   uint64_t *bufferSize);
+
+/* Replaces importNDDataset (import_dataset.cpp:34-73).  *points is
+ * allocated with malloc; release with sj_free_host. */
+SJ_API sj_status sj_import_csv(const char *path, int32_t d, double **points, uint64_t *n);
+/* Replaces sortInNDBins (import_dataset.cpp:16-30,76-80).  stable=0 uses
+ * std::sort with the reference comparator (libstdc++ order); stable=1 a
+ * stable sort. */
+SJ_API sj_status sj_sort_in_nd_bins(double *points, uint64_t n, int32_t d, int32_t stable);
+SJ_API void sj_free_host(void *p);
+
+/* Eps threshold T = max{x : sqrt_rn(x) <= eps}; sqrt(d2) <= eps iff d2 <= T. */
+SJ_API double sj_eps_threshold(double eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SJAMD_H */

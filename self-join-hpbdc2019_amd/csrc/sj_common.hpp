@@ -1,0 +1,179 @@
+// Internal declarations shared by the HIP translation units of libsjamd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sjamd.h"
+
+namespace sj {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+
+#define SJ_TRY_HIP(expr)                                                          \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::sj::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                  \
+            return SJ_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define SJ_TRY(expr)                        \
+    do {                                    \
+        sj_status s_ = (expr);              \
+        if (s_ != SJ_OK) return s_;         \
+    } while (0)
+
+#define SJ_LAUNCH_CHECK()                                                         \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            ::sj::set_error("kernel launch failed: %s (%s:%d)",                   \
+                            hipGetErrorString(e_), __FILE__, __LINE__);           \
+            return SJ_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+// ------------------------------------------------------------ device pool
+// Caching allocator: power-of-two size classes, freed blocks are kept for
+// reuse so steady-state joins make no hipMalloc calls.
+class Pool {
+  public:
+    ~Pool();
+    sj_status alloc(size_t bytes, void **out);
+    void release(void *p);
+    void trim();
+
+  private:
+    std::mutex mu_;
+    std::map<size_t, std::vector<void *>> free_;
+    std::map<void *, size_t> live_;
+};
+
+// RAII device buffer drawn from a Pool.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    Pool *pool = nullptr;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), pool(o.pool) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        reset();
+        p = o.p; n = o.n; pool = o.pool;
+        o.p = nullptr; o.n = 0;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    sj_status alloc(Pool *pl, size_t count) {
+        reset();
+        pool = pl;
+        void *q = nullptr;
+        sj_status s = pool->alloc((count ? count : 1) * sizeof(T), &q);
+        if (s != SJ_OK) return s;
+        p = static_cast<T *>(q);
+        n = count;
+        return SJ_OK;
+    }
+    void reset() {
+        if (p && pool) pool->release(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ---------------------------------------------------------- grid params
+// Written by the device grid builder, read back once by the host.
+struct GridParams {
+    double minArr[SJ_MAXK];
+    double maxArr[SJ_MAXK];
+    uint32_t nCells[SJ_MAXK];
+    uint64_t mult[SJ_MAXK];  // prod_{j<i} nCells[j]
+    uint64_t totalCells;
+    uint32_t flags;          // bit0 non-finite coordinate, bit1 overflow
+    uint32_t pad;
+};
+enum { GP_NONFINITE = 1u, GP_OVERFLOW = 2u };
+
+// Small pinned host staging area for scalar readbacks.
+struct HostStage {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// ------------------------------------------------------------- launches
+sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
+                                    uint64_t *out /* n+1 */);
+sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
+                                    uint32_t *out /* n+1 */);
+// Stable LSD radix sort of (key,val) by the low `bits` bits of key.  The
+// sorted data ends in (*keys,*vals); the other buffers are scratch.
+sj_status radix_sort_pairs_u64(Pool *pool, hipStream_t s, uint64_t **keys, uint64_t **keys_alt,
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits);
+sj_status radix_sort_pairs_u32(Pool *pool, hipStream_t s, uint32_t **keys, uint32_t **keys_alt,
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits);
+sj_status fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+sj_status fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
+
+int device_cu_count(int device);
+
+}  // namespace sj
+
+// --------------------------------------------------------------- handles
+struct sj_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    sj::Pool pool;
+    uint64_t *pinned = nullptr;  // 64 scalars of pinned readback space
+    int cu_count = 256;
+};
+
+struct sj_grid_s {
+    sj_context ctx = nullptr;
+    uint64_t n = 0;
+    int d = 0, k = 0;
+    double eps = 0.0;
+    sj::GridParams gp{};
+    uint64_t nNonEmpty = 0;
+    bool dense = false;
+    sj::DBuf<sj::GridParams> d_gp;
+    sj::DBuf<double> sortedPts;      // n x d, cell order
+    sj::DBuf<uint32_t> sortedIds;    // indexLookupArr
+    sj::DBuf<uint32_t> cellOf;       // cell of each sorted position
+    sj::DBuf<uint32_t> cellStart;    // nNonEmpty + 1
+    sj::DBuf<uint64_t> cellLinear;   // nNonEmpty
+    sj::DBuf<uint32_t> cellCoord;    // nNonEmpty x k
+    sj::DBuf<int32_t> denseCell;     // totalCells (dense lookup) or empty
+    sj::DBuf<float> sortedPts32;     // lazily built for the FP32 form
+    double ms_build = 0.0;
+};
+
+struct sj_batch_host {
+    std::vector<int32_t> ids;          // neighbour IDs of the batch rows
+    std::vector<uint32_t> rowIds;      // original IDs of the rows
+    std::vector<uint64_t> rowOffsets;  // rows+1, into ids
+};
+
+struct sj_result_s {
+    sj_context ctx = nullptr;
+    uint64_t n = 0;
+    uint64_t total = 0;
+    bool on_device = false;
+    sj::DBuf<uint64_t> d_offsets;  // n+1, original ID order
+    sj::DBuf<int32_t> d_ids;       // total
+    std::vector<sj_batch_host> batches;
+    sj_stats stats{};
+};

@@ -1,0 +1,348 @@
+// Device grid construction (reference main.cu:240-503, built on the host
+// there).  Steps, all on the context stream:
+//   1. per-dimension min/max + finiteness check        (k_minmax)
+//   2. grid parameters: min-eps, max+eps, nCells, ID multipliers
+//                                                       (k_grid_params)
+//   3. per-point cell coordinates -> linear cell ID    (k_cell_keys)
+//   4. stable LSD radix sort of (linearID, pointID)    (points stay in
+//      ascending ID inside a cell, as main.cu:291-313 produces)
+//   5. head flags + exclusive scan -> compacted non-empty cell table
+//   6. coordinates gathered into cell order, per-cell coordinates and the
+//      O(1) dense lookup table when the grid is small enough.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "sj_common.hpp"
+
+namespace sj {
+
+constexpr int GB = 256;
+
+__global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, int d, int k,
+                                               double *partial, uint32_t *flags) {
+    double mn[SJ_MAXK], mx[SJ_MAXK];
+#pragma unroll
+    for (int j = 0; j < SJ_MAXK; j++) {
+        mn[j] = INFINITY;
+        mx[j] = -INFINITY;
+    }
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * GB) {
+        const double *p = pts + i * (uint64_t)d;
+        for (int j = 0; j < d; j++) {
+            double x = p[j];
+            bad |= !isfinite(x);
+        }
+#pragma unroll
+        for (int j = 0; j < SJ_MAXK; j++) {
+            if (j < k) {
+                double x = p[j];
+                mn[j] = fmin(mn[j], x);
+                mx[j] = fmax(mx[j], x);
+            }
+        }
+    }
+    if (bad) atomicOr(flags, (uint32_t)GP_NONFINITE);
+    __shared__ double smn[GB / 64][SJ_MAXK], smx[GB / 64][SJ_MAXK];
+    const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < SJ_MAXK; j++) {
+        double a = mn[j], b = mx[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a = fmin(a, __shfl_xor(a, off, 64));
+            b = fmax(b, __shfl_xor(b, off, 64));
+        }
+        if (lane == 0) {
+            smn[wave][j] = a;
+            smx[wave][j] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)k) {
+        int j = threadIdx.x;
+        double a = INFINITY, b = -INFINITY;
+        for (int w = 0; w < GB / 64; w++) {
+            a = fmin(a, smn[w][j]);
+            b = fmax(b, smx[w][j]);
+        }
+        partial[(uint64_t)blockIdx.x * 2 * k + j] = a;
+        partial[(uint64_t)blockIdx.x * 2 * k + k + j] = b;
+    }
+}
+
+// main.cu:454-501: min/max, then minArr -= eps, maxArr += eps,
+// nCells = ceil((max-min)/eps), totalCells = prod nCells (dim 0 fastest).
+__global__ void k_grid_params(const double *partial, int nb, int k, double eps, GridParams *gp,
+                              const uint32_t *flags) {
+    if (threadIdx.x != 0) return;
+    uint32_t f = *flags;
+    uint64_t tot = 1, mul = 1;
+    for (int j = 0; j < k; j++) {
+        double a = INFINITY, b = -INFINITY;
+        for (int i = 0; i < nb; i++) {
+            a = fmin(a, partial[(uint64_t)i * 2 * k + j]);
+            b = fmax(b, partial[(uint64_t)i * 2 * k + k + j]);
+        }
+        a -= eps;
+        b += eps;
+        gp->minArr[j] = a;
+        gp->maxArr[j] = b;
+        double c = ceil((b - a) / eps);
+        uint32_t nc = 0;
+        if (!(c < 4294967296.0) || !(c >= 1.0))
+            f |= GP_OVERFLOW;
+        else
+            nc = (uint32_t)c;
+        gp->nCells[j] = nc;
+        gp->mult[j] = mul;
+        unsigned __int128 t = (unsigned __int128)tot * nc;
+        if (t >> 64) f |= GP_OVERFLOW;
+        tot = (uint64_t)t;
+        mul = tot;
+    }
+    for (int j = k; j < SJ_MAXK; j++) {
+        gp->minArr[j] = 0;
+        gp->maxArr[j] = 0;
+        gp->nCells[j] = 1;
+        gp->mult[j] = 0;
+    }
+    gp->totalCells = tot;
+    gp->flags = f;
+    gp->pad = 0;
+}
+
+// Cell coordinate (unsigned)((x - minArr)/eps): FP64 subtract, correctly
+// rounded divide, truncation (main.cu:263,294; kernel.cu:1457).
+template <class KT>
+__global__ __launch_bounds__(GB) void k_cell_keys(const double *pts, uint64_t n, int d, int k,
+                                                  double eps, const GridParams *gp, KT *keys,
+                                                  uint32_t *vals, uint32_t *flags) {
+    uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= n) return;
+    const double *p = pts + i * (uint64_t)d;
+    uint64_t lin = 0;
+    bool bad = false;
+    for (int j = 0; j < k; j++) {
+        double v = (p[j] - gp->minArr[j]) / eps;
+        uint32_t c = 0;
+        if (!(v >= 0.0) || !(v < (double)gp->nCells[j]))
+            bad = true;
+        else
+            c = (uint32_t)v;
+        lin += (uint64_t)c * gp->mult[j];
+    }
+    if (bad) atomicOr(flags, (uint32_t)GP_OVERFLOW);
+    keys[i] = (KT)lin;
+    vals[i] = (uint32_t)i;
+}
+
+template <class KT>
+__global__ __launch_bounds__(GB) void k_head_flags(const KT *keys, uint64_t n, uint32_t *head) {
+    uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+template <class KT>
+__global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint32_t *head,
+                                                      const uint32_t *ex, uint64_t n,
+                                                      uint32_t *cellOf, uint32_t *cellStart,
+                                                      uint64_t *cellLinear) {
+    uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = ex[i] + head[i] - 1u;
+    cellOf[i] = c;
+    if (head[i]) {
+        cellStart[c] = (uint32_t)i;
+        cellLinear[c] = (uint64_t)keys[i];
+    }
+    if (i == n - 1) cellStart[c + 1] = (uint32_t)n;
+}
+
+__global__ __launch_bounds__(GB) void k_gather_points(const double *pts, const uint32_t *ids,
+                                                      uint64_t n, int d, double *out) {
+    uint64_t e = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    uint64_t total = n * (uint64_t)d;
+    if (e >= total) return;
+    uint64_t s = e / (uint64_t)d;
+    int j = (int)(e - s * (uint64_t)d);
+    out[e] = pts[(uint64_t)ids[s] * d + j];
+}
+
+__global__ __launch_bounds__(GB) void k_cell_coords(const double *spts, const uint32_t *cellStart,
+                                                    uint64_t nne, int d, int k, double eps,
+                                                    const GridParams *gp, uint32_t *coord) {
+    uint64_t c = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (c >= nne) return;
+    const double *p = spts + (uint64_t)cellStart[c] * d;
+    for (int j = 0; j < k; j++) coord[c * k + j] = (uint32_t)((p[j] - gp->minArr[j]) / eps);
+}
+
+__global__ __launch_bounds__(GB) void k_dense_scatter(const uint64_t *cellLinear, uint64_t nne,
+                                                      int32_t *dense) {
+    uint64_t c = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (c >= nne) return;
+    dense[cellLinear[c]] = (int32_t)c;
+}
+
+static unsigned blocks_for(uint64_t n, unsigned b = GB) {
+    uint64_t g = (n + b - 1) / b;
+    return (unsigned)(g ? g : 1);
+}
+
+static int bits_for(uint64_t maxv) {
+    int b = 0;
+    while (b < 64 && (maxv >> b)) b++;
+    return b;
+}
+
+template <class KT>
+static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t *d_flags) {
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    const uint64_t n = g->n;
+    DBuf<KT> keys, keys2;
+    DBuf<uint32_t> vals, vals2;
+    SJ_TRY(keys.alloc(pool, n));
+    SJ_TRY(keys2.alloc(pool, n));
+    SJ_TRY(vals.alloc(pool, n));
+    SJ_TRY(vals2.alloc(pool, n));
+    hipLaunchKernelGGL((k_cell_keys<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, d_points, n, g->d, g->k,
+                       g->eps, g->d_gp.p, keys.p, vals.p, d_flags);
+    SJ_LAUNCH_CHECK();
+    KT *kp = keys.p, *ka = keys2.p;
+    uint32_t *vp = vals.p, *va = vals2.p;
+    const int bits = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
+    sj_status st;
+    if constexpr (sizeof(KT) == 8)
+        st = radix_sort_pairs_u64(pool, s, &kp, &ka, &vp, &va, n, bits);
+    else
+        st = radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, n, bits);
+    SJ_TRY(st);
+
+    DBuf<uint32_t> head, ex;
+    SJ_TRY(head.alloc(pool, n));
+    SJ_TRY(ex.alloc(pool, n + 1));
+    hipLaunchKernelGGL((k_head_flags<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, n, head.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY(scan_exclusive_u32_to_u32(pool, s, head.p, n, ex.p));
+    SJ_TRY(g->cellOf.alloc(pool, n));
+    SJ_TRY(g->cellStart.alloc(pool, n + 1));
+    SJ_TRY(g->cellLinear.alloc(pool, n));
+    hipLaunchKernelGGL((k_compact_cells<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, head.p, ex.p, n,
+                       g->cellOf.p, g->cellStart.p, g->cellLinear.p);
+    SJ_LAUNCH_CHECK();
+    // sorted point IDs (indexLookupArr)
+    SJ_TRY(g->sortedIds.alloc(pool, n));
+    SJ_TRY_HIP(hipMemcpyAsync(g->sortedIds.p, vp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    // nNonEmpty and the error flags
+    uint64_t *pin = g->ctx->pinned;
+    SJ_TRY_HIP(hipMemcpyAsync(pin, ex.p + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipMemcpyAsync(pin + 1, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipStreamSynchronize(s));
+    uint32_t nne = (uint32_t)(pin[0] & 0xffffffffu);
+    uint32_t fl = (uint32_t)(pin[1] & 0xffffffffu);
+    if (fl & GP_OVERFLOW) {
+        set_error("cell coordinate outside the grid (overflow)");
+        return SJ_ERR_OVERFLOW;
+    }
+    g->nNonEmpty = nne;
+    return SJ_OK;
+}
+
+sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, int k, double eps,
+                     sj_grid_s *g) {
+    if (!ctx || !d_points || n == 0 || d < 1 || d > SJ_MAXD || k < 1 || k > SJ_MAXK || k > d) {
+        set_error("grid_build: invalid arguments (n=%llu d=%d k=%d)", (unsigned long long)n, d, k);
+        return SJ_ERR_INVALID;
+    }
+    if (!(eps > 0.0) || std::isinf(eps)) {
+        set_error("grid_build: eps must be finite and > 0");
+        return SJ_ERR_INVALID;
+    }
+    if (n >= 0x7fffffffull) {
+        set_error("grid_build: at most 2^31-2 points (int32 point IDs)");
+        return SJ_ERR_OVERFLOW;
+    }
+    Pool *pool = &ctx->pool;
+    hipStream_t s = ctx->stream;
+    g->ctx = ctx;
+    g->n = n;
+    g->d = d;
+    g->k = k;
+    g->eps = eps;
+
+    const unsigned nbmm = (unsigned)std::min<uint64_t>(1024, blocks_for(n));
+    DBuf<double> partial;
+    DBuf<uint32_t> flags;
+    SJ_TRY(partial.alloc(pool, (uint64_t)nbmm * 2 * k));
+    SJ_TRY(flags.alloc(pool, 1));
+    SJ_TRY(g->d_gp.alloc(pool, 1));
+    SJ_TRY(fill_u32(s, flags.p, 1, 0u));
+    hipLaunchKernelGGL(k_minmax, dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);
+    SJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(64), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
+                       flags.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY_HIP(hipMemcpyAsync(&g->gp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipStreamSynchronize(s));
+    if (g->gp.flags & GP_NONFINITE) {
+        set_error("grid_build: non-finite coordinate");
+        return SJ_ERR_INVALID;
+    }
+    if (g->gp.flags & GP_OVERFLOW) {
+        set_error("grid_build: cell count overflows (nCells >= 2^32 or totalCells >= 2^64)");
+        return SJ_ERR_OVERFLOW;
+    }
+
+    if (g->gp.totalCells <= 0xFFFFFFFFull)
+        SJ_TRY(sort_and_compact<uint32_t>(g, d_points, flags.p));
+    else
+        SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags.p));
+
+    const uint64_t nne = g->nNonEmpty;
+    SJ_TRY(g->sortedPts.alloc(pool, n * (uint64_t)d));
+    hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n * (uint64_t)d)), dim3(GB), 0, s, d_points,
+                       g->sortedIds.p, n, d, g->sortedPts.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY(g->cellCoord.alloc(pool, nne * (uint64_t)k));
+    hipLaunchKernelGGL(k_cell_coords, dim3(blocks_for(nne)), dim3(GB), 0, s, g->sortedPts.p,
+                       g->cellStart.p, nne, d, k, eps, g->d_gp.p, g->cellCoord.p);
+    SJ_LAUNCH_CHECK();
+
+    // O(1) lookup when the full cell table is at most 64M entries or 16x n
+    const uint64_t tc = g->gp.totalCells;
+    g->dense = tc <= (1ull << 26) || tc <= 16ull * n;
+    if (g->dense) {
+        SJ_TRY(g->denseCell.alloc(pool, tc));
+        SJ_TRY(fill_u32(s, reinterpret_cast<uint32_t *>(g->denseCell.p), tc, 0xFFFFFFFFu));
+        hipLaunchKernelGGL(k_dense_scatter, dim3(blocks_for(nne)), dim3(GB), 0, s, g->cellLinear.p, nne,
+                           g->denseCell.p);
+        SJ_LAUNCH_CHECK();
+    }
+    return SJ_OK;
+}
+
+// Per-dimension occupied-coordinate mask (main.cu:381-401), host side.
+sj_status grid_nd_mask_host(sj_grid_s *g, std::vector<uint32_t> &mask, std::vector<uint32_t> &offsets) {
+    const uint64_t nne = g->nNonEmpty;
+    const int k = g->k;
+    std::vector<uint32_t> coord(nne * k);
+    SJ_TRY_HIP(hipMemcpy(coord.data(), g->cellCoord.p, coord.size() * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    mask.clear();
+    offsets.assign(2 * k, 0);
+    for (int j = 0; j < k; j++) {
+        std::vector<uint8_t> occ(g->gp.nCells[j], 0);
+        for (uint64_t c = 0; c < nne; c++) occ[coord[c * k + j]] = 1;
+        offsets[2 * j] = (uint32_t)mask.size();
+        for (uint32_t v = 0; v < g->gp.nCells[j]; v++)
+            if (occ[v]) mask.push_back(v);
+        offsets[2 * j + 1] = (uint32_t)mask.size() - 1;
+    }
+    return SJ_OK;
+}
+
+}  // namespace sj

@@ -1,0 +1,354 @@
+"""Python view of libsjamd, the MI355X grid-indexed epsilon self-join.
+
+Thin ctypes layer over the C ABI in include/sjamd.h.  Device buffers come
+from torch (ROCm) tensors; the library does all the work in HIP kernels.
+Loading fails loudly when the native library is missing: there is no
+CPU fallback in this package.
+
+Reference interface map (benoitgallet/self-join-hpbdc2019):
+  Context            warmUpGPU                 GPU.h:54
+  Context.grid_build generateNDGridDimensions  main.cu:446-503
+                     populateNDGridIndexAndLookupArray  main.cu:240-413
+  Grid.join          distanceTableNDGridBatches GPU.h:37-40
+  Result.neighbor_table constructNeighborTableKeyValueWithPtrs GPU.cu:2595
+  Grid.workload      sortByWorkLoad[LidUnicomp] kernel.cu:33-250
+  Grid.batch_estimate callGPUBatchEst[WorkQueue] GPU.cu:854-1284
+  import_csv         importNDDataset           import_dataset.cpp:34
+  sort_in_nd_bins    sortInNDBins              import_dataset.cpp:76
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsjamd.so")
+MAXK = 12
+
+FULL, UNICOMP, BUNICOMP, LID = 0, 1, 2, 3
+FP64, FP32_RECHECK = 0, 1
+STRIDED, WORKQUEUE = 0, 1
+
+FORMS = {"full": FULL, "unicomp": UNICOMP, "bunicomp": BUNICOMP, "lid": LID}
+
+# reference search modes (main.cu:88-150) -> (form, schedule)
+SEARCH_MODES = {
+    3: (FULL, STRIDED),
+    4: (UNICOMP, STRIDED),
+    5: (BUNICOMP, STRIDED),
+    6: (LID, STRIDED),
+    7: (FULL, WORKQUEUE),
+    9: (FULL, WORKQUEUE),
+    10: (LID, WORKQUEUE),
+}
+
+
+class SJError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"sjamd status {status}: {msg}")
+        self.status = status
+
+
+class JoinParams(C.Structure):
+    _fields_ = [
+        ("form", C.c_int32), ("precision", C.c_int32), ("schedule", C.c_int32),
+        ("sort_rows", C.c_int32), ("capacity", C.c_uint64), ("block_size", C.c_int32),
+        ("queue_chunk", C.c_int32), ("cells_per_tile", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64), ("d", C.c_int32), ("k", C.c_int32), ("eps", C.c_double),
+        ("minArr", C.c_double * MAXK), ("maxArr", C.c_double * MAXK),
+        ("nCells", C.c_uint32 * MAXK), ("totalCells", C.c_uint64), ("nNonEmpty", C.c_uint64),
+        ("dense_lookup", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("total_pairs", C.c_uint64), ("num_batches", C.c_uint64), ("candidates", C.c_uint64),
+        ("fp32_band_pairs", C.c_uint64), ("fp32_misclassified", C.c_uint64),
+        ("overflow_retries", C.c_uint64),
+        ("ms_grid", C.c_double), ("ms_workload", C.c_double), ("ms_count", C.c_double),
+        ("ms_scan", C.c_double), ("ms_fill", C.c_double), ("ms_sort_rows", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class NeighborLookup(C.Structure):
+    _fields_ = [("pointID", C.c_int32), ("indexmin", C.c_int32), ("indexmax", C.c_int32),
+                ("batch", C.c_int32)]
+
+
+EXPORTS = [
+    "sj_last_error", "sj_context_create", "sj_context_destroy", "sj_context_set_stream",
+    "sj_grid_build", "sj_grid_get_info", "sj_grid_copy_arrays", "sj_grid_nd_mask", "sj_grid_free",
+    "sj_join", "sj_self_join", "sj_result_get_stats", "sj_result_total_pairs",
+    "sj_result_num_points", "sj_result_device_csr", "sj_result_copy_csr", "sj_result_copy_pairs",
+    "sj_result_neighbor_table", "sj_result_batch_data", "sj_result_free", "sj_workload",
+    "sj_batch_estimate", "sj_import_csv", "sj_sort_in_nd_bins", "sj_free_host", "sj_eps_threshold",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libsjamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libsjamd.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int32
+    P = C.POINTER
+    L.sj_last_error.restype = C.c_char_p
+    L.sj_eps_threshold.restype = C.c_double
+    L.sj_eps_threshold.argtypes = [C.c_double]
+    L.sj_context_create.argtypes = [i32, vp, P(vp)]
+    L.sj_context_destroy.argtypes = [vp]
+    L.sj_context_destroy.restype = None
+    L.sj_context_set_stream.argtypes = [vp, vp]
+    L.sj_grid_build.argtypes = [vp, vp, u64, i32, i32, C.c_double, P(vp)]
+    L.sj_grid_get_info.argtypes = [vp, P(GridInfo)]
+    L.sj_grid_copy_arrays.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.sj_grid_nd_mask.argtypes = [vp, vp, vp, P(u64)]
+    L.sj_grid_free.argtypes = [vp]
+    L.sj_grid_free.restype = None
+    L.sj_join.argtypes = [vp, P(JoinParams), P(vp)]
+    L.sj_self_join.argtypes = [vp, vp, u64, i32, i32, C.c_double, P(JoinParams), P(vp)]
+    L.sj_result_get_stats.argtypes = [vp, P(Stats)]
+    L.sj_result_total_pairs.argtypes = [vp]
+    L.sj_result_total_pairs.restype = u64
+    L.sj_result_num_points.argtypes = [vp]
+    L.sj_result_num_points.restype = u64
+    L.sj_result_device_csr.argtypes = [vp, P(vp), P(vp)]
+    L.sj_result_copy_csr.argtypes = [vp, vp, vp]
+    L.sj_result_copy_pairs.argtypes = [vp, vp, vp]
+    L.sj_result_neighbor_table.argtypes = [vp, vp]
+    L.sj_result_batch_data.argtypes = [vp, u64, P(vp), P(u64)]
+    L.sj_result_free.argtypes = [vp]
+    L.sj_result_free.restype = None
+    L.sj_workload.argtypes = [vp, i32, vp, vp]
+    L.sj_batch_estimate.argtypes = [vp, u64, P(u64), P(u64), P(u64), P(u64)]
+    L.sj_import_csv.argtypes = [C.c_char_p, i32, P(P(C.c_double)), P(u64)]
+    L.sj_sort_in_nd_bins.argtypes = [P(C.c_double), u64, i32, i32]
+    L.sj_free_host.argtypes = [vp]
+    L.sj_free_host.restype = None
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().sj_last_error()
+        raise SJError(status, msg.decode() if msg else "")
+
+
+def _np_ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def eps_threshold(eps):
+    return lib().sj_eps_threshold(float(eps))
+
+
+def make_params(form=LID, precision=FP64, schedule=STRIDED, sort_rows=False, capacity=0,
+                queue_chunk=0):
+    if isinstance(form, str):
+        form = FORMS[form]
+    return JoinParams(form=form, precision=precision, schedule=schedule,
+                      sort_rows=1 if sort_rows else 0, capacity=capacity, block_size=0,
+                      queue_chunk=queue_chunk, cells_per_tile=0, reserved=0)
+
+
+class Context:
+    """Device, stream and caching device pool (replaces warmUpGPU)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+
+        self.torch = torch
+        self.device = device
+        self.stream = stream  # a torch.cuda.Stream or None
+        h = C.c_void_p()
+        sp = C.c_void_p(stream.cuda_stream) if stream is not None else C.c_void_p()
+        _check(lib().sj_context_create(device, sp, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().sj_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def to_device(self, points):
+        """float64 (n, d) array -> contiguous device tensor."""
+        torch = self.torch
+        if isinstance(points, torch.Tensor):
+            t = points
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(points, dtype=np.float64))
+        return t.to(device=f"cuda:{self.device}", dtype=torch.float64).contiguous()
+
+    def grid_build(self, points, k, eps):
+        t = self.to_device(points)
+        n, d = t.shape
+        h = C.c_void_p()
+        _check(lib().sj_grid_build(self.h, C.c_void_p(t.data_ptr()), n, d, k, float(eps),
+                                   C.byref(h)))
+        return Grid(self, h, t)
+
+    def self_join(self, points, k, eps, **kw):
+        t = self.to_device(points)
+        n, d = t.shape
+        p = make_params(**kw)
+        h = C.c_void_p()
+        _check(lib().sj_self_join(self.h, C.c_void_p(t.data_ptr()), n, d, k, float(eps),
+                                  C.byref(p), C.byref(h)))
+        return Result(self, h)
+
+
+class Grid:
+    def __init__(self, ctx, h, tensor):
+        self.ctx, self.h, self._t = ctx, h, tensor
+        gi = GridInfo()
+        _check(lib().sj_grid_get_info(h, C.byref(gi)))
+        self.info = gi
+        self.n, self.d, self.k = gi.n, gi.d, gi.k
+        self.nNonEmpty = gi.nNonEmpty
+
+    def free(self):
+        if self.h:
+            lib().sj_grid_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def dims(self):
+        k = self.k
+        gi = self.info
+        return (np.array(gi.minArr[:k]), np.array(gi.maxArr[:k]),
+                np.array(gi.nCells[:k], dtype=np.uint32), gi.totalCells)
+
+    def arrays(self):
+        nne, n, k = self.nNonEmpty, self.n, self.k
+        lin = np.zeros(nne, dtype=np.uint64)
+        imin = np.zeros(nne, dtype=np.int32)
+        imax = np.zeros(nne, dtype=np.int32)
+        look = np.zeros(n, dtype=np.uint32)
+        coord = np.zeros(nne * k, dtype=np.uint32)
+        _check(lib().sj_grid_copy_arrays(self.h, _np_ptr(lin), _np_ptr(imin), _np_ptr(imax),
+                                         _np_ptr(look), _np_ptr(coord)))
+        return dict(cellLinear=lin, indexMin=imin, indexMax=imax, indexLookup=look,
+                    cellCoord=coord.reshape(nne, k))
+
+    def nd_mask(self):
+        ne = C.c_uint64()
+        _check(lib().sj_grid_nd_mask(self.h, None, None, C.byref(ne)))
+        mask = np.zeros(ne.value, dtype=np.uint32)
+        offs = np.zeros(2 * self.k, dtype=np.uint32)
+        _check(lib().sj_grid_nd_mask(self.h, _np_ptr(mask), _np_ptr(offs), C.byref(ne)))
+        return mask, offs
+
+    def join(self, **kw):
+        p = make_params(**kw)
+        h = C.c_void_p()
+        _check(lib().sj_join(self.h, C.byref(p), C.byref(h)))
+        return Result(self.ctx, h)
+
+    def workload(self, lid=False):
+        w = np.zeros(self.nNonEmpty, dtype=np.int64)
+        o = np.zeros(self.nNonEmpty, dtype=np.uint32)
+        _check(lib().sj_workload(self.h, 1 if lid else 0, _np_ptr(w), _np_ptr(o)))
+        return w, o
+
+    def batch_estimate(self, stride=100):
+        vals = [C.c_uint64() for _ in range(4)]
+        _check(lib().sj_batch_estimate(self.h, stride, *[C.byref(v) for v in vals]))
+        cnt, est, nb, buf = (v.value for v in vals)
+        return dict(sample_count=cnt, estimated=est, num_batches=nb, buffer_size=buf)
+
+
+class Result:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        self.total = lib().sj_result_total_pairs(h)
+        self.n = lib().sj_result_num_points(h)
+
+    def free(self):
+        if self.h:
+            lib().sj_result_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        _check(lib().sj_result_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def device_csr(self):
+        o, i = C.c_void_p(), C.c_void_p()
+        _check(lib().sj_result_device_csr(self.h, C.byref(o), C.byref(i)))
+        return o.value, i.value
+
+    def csr(self):
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        ids = np.zeros(max(self.total, 1), dtype=np.int32)
+        _check(lib().sj_result_copy_csr(self.h, _np_ptr(off), _np_ptr(ids)))
+        return off, ids[:self.total]
+
+    def pairs(self):
+        k = np.zeros(max(self.total, 1), dtype=np.int32)
+        v = np.zeros(max(self.total, 1), dtype=np.int32)
+        _check(lib().sj_result_copy_pairs(self.h, _np_ptr(k), _np_ptr(v)))
+        return k[:self.total], v[:self.total]
+
+    def neighbor_table(self):
+        tab = np.zeros((self.n, 4), dtype=np.int32)
+        _check(lib().sj_result_neighbor_table(self.h, _np_ptr(tab)))
+        return tab
+
+    def batch_data(self, b):
+        p, cnt = C.c_void_p(), C.c_uint64()
+        _check(lib().sj_result_batch_data(self.h, b, C.byref(p), C.byref(cnt)))
+        if cnt.value == 0:
+            return np.zeros(0, dtype=np.int32)
+        arr = (C.c_int32 * cnt.value).from_address(p.value)
+        return np.frombuffer(arr, dtype=np.int32).copy()
+
+
+def import_csv(path, d):
+    ptr = C.POINTER(C.c_double)()
+    n = C.c_uint64()
+    _check(lib().sj_import_csv(str(path).encode(), d, C.byref(ptr), C.byref(n)))
+    cnt = n.value * d
+    out = np.ctypeslib.as_array(ptr, (max(cnt, 1),))[:cnt].copy().reshape(n.value, d)
+    lib().sj_free_host(C.cast(ptr, C.c_void_p))
+    return out
+
+
+def sort_in_nd_bins(points, stable=False):
+    p = np.ascontiguousarray(points, dtype=np.float64).copy()
+    _check(lib().sj_sort_in_nd_bins(p.ctypes.data_as(C.POINTER(C.c_double)), p.shape[0],
+                                    p.shape[1], 1 if stable else 0))
+    return p
