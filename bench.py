@@ -1,0 +1,194 @@
+"""Self-join benchmark: result pairs/s and join time at 2M points (2-D,
+Exp(40) clipped to [0,1], seed 2, eps for ~64 neighbours per point),
+workload sort + work queue + B-Unicomp, FP32+FP64-recheck vs plain FP64.
+
+One step = grid construction + count pass + offsets + fill pass over the
+2,000,000 points already resident in HBM, producing the exact neighbour
+table (CSR, original point IDs) in HBM.
+
+Prints one JSON line (rank 0).  The roofline object covers the dominant
+kernel, timed with HIP events on the stream it runs on; cpu_baseline is the
+CPU oracle (grid build + LID join, OpenMP) on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "self-join-hpbdc2019_amd"))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "exp2d"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--form", default="bunicomp")
+    ap.add_argument("--schedule", default="workqueue")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def run_gpu(ctx, x, k, eps, prec, form, sched, steps, warmup, torch):
+    import sjamd
+
+    n, d = x.shape
+    kw = dict(form=form, precision=prec, schedule=sched)
+    stats = []
+    for _ in range(warmup):
+        r = ctx.self_join(x, k, eps, **kw)
+        r.free()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    total = None
+    for _ in range(steps):
+        r = ctx.self_join(x, k, eps, **kw)
+        stats.append(r.stats())
+        total = r.total
+        r.free()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    avg = {key: sum(s[key] for s in stats) / len(stats) for key in stats[0]}
+    return dt, total, avg
+
+
+def checksum(ctx, x, k, eps, prec, form, sched):
+    """Order-free fingerprint of the neighbour table: row lengths and the
+    per-row sum of IDs, reduced on the device."""
+    import torch
+
+    r = ctx.self_join(x, k, eps, form=form, precision=prec, schedule=sched)
+    off_p, ids_p = r.device_csr()
+    n = x.shape[0]
+    off = torch.empty(n + 1, dtype=torch.int64, device=x.device)
+    ids = torch.empty(max(r.total, 1), dtype=torch.int32, device=x.device)
+    torch.cuda.synchronize()
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy(ctypes.c_void_p(off.data_ptr()), ctypes.c_void_p(off_p), (n + 1) * 8, 3)
+    if r.total:
+        hip.hipMemcpy(ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(ids_p), r.total * 4, 3)
+    rows = torch.repeat_interleave(torch.arange(n, device=x.device), off[1:] - off[:-1])
+    v = ids[:r.total].to(torch.int64)
+    fp = (int(off[-1].item()), int((v * (rows + 1)).sum().item()), int((v * v).sum().item()))
+    r.free()
+    return fp
+
+
+def cpu_baseline(points, k, eps, reps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        cores = min(cores, int(env))
+    best = None
+    total = 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = O.OracleGrid(points, k, eps)
+        off, ids = g.join(O.LID, nthreads=cores, sort_rows=False)
+        dt = time.perf_counter() - t0
+        total = int(off[-1])
+        del g, off, ids
+        best = dt if best is None else min(best, dt)
+    return {"value": total / best, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"full workload ({points.shape[0]} points, grid build + LID join), "
+                      f"best of {reps}", "seconds": best}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the join shards by query cells; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import sjamd
+    from sjamd import datasets as D
+
+    w = D.WORKLOADS[a.workload]
+    pts = D.make_points(a.workload)
+    eps = D.workload_eps(a.workload)
+    k = w["k"]
+    n, d = pts.shape
+    form = sjamd.FORMS[a.form]
+    sched = sjamd.WORKQUEUE if a.schedule == "workqueue" else sjamd.STRIDED
+
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx = sjamd.Context(0, stream=stream)
+        x = ctx.to_device(pts)
+        torch.cuda.synchronize()
+        fp64 = run_gpu(ctx, x, k, eps, sjamd.FP64, form, sched, a.steps, a.warmup, torch)
+        fp32 = run_gpu(ctx, x, k, eps, sjamd.FP32_RECHECK, form, sched, a.steps, a.warmup, torch)
+        fp_a = checksum(ctx, x, k, eps, sjamd.FP64, form, sched)
+        fp_b = checksum(ctx, x, k, eps, sjamd.FP32_RECHECK, form, sched)
+    exact = fp_a == fp_b
+
+    dt, total, st = fp32
+    ms = dt * 1e3
+    # dominant kernel: the longer of the two traversal passes
+    b_alg_fill = n * d * 8 + total * 4 + (n + 1) * 8
+    b_alg_count = n * d * 8 + n * 4 * 2
+    if st["ms_fill"] >= st["ms_count"]:
+        kname, kms, kbytes = "fill", st["ms_fill"], b_alg_fill
+    else:
+        kname, kms, kbytes = "count", st["ms_count"], b_alg_count
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_{kname}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "result pairs/sec & join time at 2M pts, FP32+recheck vs FP64, % HBM roofline",
+        "value": total / dt,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {n} points x {d}-D, Exp(40) clipped [0,1], seed 2, "
+                               f"eps={eps.hex()} (~64 neighbours/pt)",
+                   "form": a.form, "schedule": a.schedule, "precision": "fp32+fp64-recheck",
+                   "pairs": total, "exact_vs_fp64": exact},
+        "fp64": {"ms_per_step": fp64[0] * 1e3, "pairs_per_s": fp64[1] / fp64[0],
+                 "ms_count": fp64[2]["ms_count"], "ms_fill": fp64[2]["ms_fill"]},
+        "fp32_recheck": {"ms_per_step": ms, "ms_grid": st["ms_grid"], "ms_workload": st["ms_workload"],
+                         "ms_count": st["ms_count"], "ms_scan": st["ms_scan"], "ms_fill": st["ms_fill"],
+                         "band_pairs": st["fp32_band_pairs"],
+                         "misclassified_before_recheck": st["fp32_misclassified"],
+                         "candidates": st["candidates"]},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes": kbytes, "kernel_ms": kms},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pts, k, eps, a.cpu_reps)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
