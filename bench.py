@@ -60,24 +60,20 @@ def run_gpu(ctx, x, k, eps, prec, form, sched, steps, warmup, torch):
 
 
 def checksum(ctx, x, k, eps, prec, form, sched):
-    """Order-free fingerprint of the neighbour table: row lengths and the
-    per-row sum of IDs, reduced on the device."""
+    """Order-free fingerprint of the neighbour table: |S|, sum of
+    (row ID + 1) * neighbour ID and sum of squared IDs, on the device."""
     import torch
 
-    r = ctx.self_join(x, k, eps, form=form, precision=prec, schedule=sched)
-    off_p, ids_p = r.device_csr()
-    n = x.shape[0]
-    off = torch.empty(n + 1, dtype=torch.int64, device=x.device)
-    ids = torch.empty(max(r.total, 1), dtype=torch.int32, device=x.device)
-    torch.cuda.synchronize()
-    import ctypes
+    import sjamd
 
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy(ctypes.c_void_p(off.data_ptr()), ctypes.c_void_p(off_p), (n + 1) * 8, 3)
-    if r.total:
-        hip.hipMemcpy(ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(ids_p), r.total * 4, 3)
-    rows = torch.repeat_interleave(torch.arange(n, device=x.device), off[1:] - off[:-1])
-    v = ids[:r.total].to(torch.int64)
+    r = ctx.self_join(x, k, eps, form=form, precision=prec, schedule=sched)
+    off_p, ids_p, rid_p = r.device_table()
+    n = x.shape[0]
+    off = torch.from_numpy(sjamd.download(off_p, n + 1, "int64")).to(x.device)
+    ids = torch.from_numpy(sjamd.download(ids_p, r.total, "int32")).to(x.device)
+    rid = torch.from_numpy(sjamd.download(rid_p, n, "int32")).to(x.device)
+    rows = torch.repeat_interleave(rid.to(torch.int64), off[1:] - off[:-1])
+    v = ids.to(torch.int64)
     fp = (int(off[-1].item()), int((v * (rows + 1)).sum().item()), int((v * v).sum().item()))
     r.free()
     return fp

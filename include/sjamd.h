@@ -147,8 +147,15 @@ SJ_API sj_status sj_self_join(sj_context ctx, const double *d_points, uint64_t n
 SJ_API sj_status sj_result_get_stats(sj_result r, sj_stats *stats);
 SJ_API uint64_t sj_result_total_pairs(sj_result r);
 SJ_API uint64_t sj_result_num_points(sj_result r);
-/* Device CSR in original point-ID order: offsets[n+1] (uint64), ids[|S|].
- * Available when the result is a single device-resident batch. */
+/* Device neighbour table as produced (no copy): rows in cell order; row r
+ * lists the neighbours of point rowIds[r] in ids[offsets[r]..offsets[r+1]).
+ * This is the device form of the reference's neighbour table
+ * (neighborTableLookup + data arrays, structs.h:41-57).  Available when the
+ * result is a single device-resident batch. */
+SJ_API sj_status sj_result_device_table(sj_result r, const uint64_t **d_offsets,
+                                        const int32_t **d_ids, const uint32_t **d_rowIds);
+/* Device CSR in original point-ID order: offsets[n+1] (uint64), ids[|S|],
+ * built from the table on the first call (one extra pass over the pairs). */
 SJ_API sj_status sj_result_device_csr(sj_result r, const uint64_t **d_offsets,
                                       const int32_t **d_ids);
 /* Host copy of the CSR (any batching). */

@@ -18,6 +18,7 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count);
 double eps_threshold(double eps);
 sj_status import_csv(const char *path, int d, double **pts, uint64_t *n);
 sj_status sort_in_nd_bins(double *pts, uint64_t n, int d, int stable);
+sj_status result_to_orig(sj_result_s *r);
 }  // namespace sj
 
 using namespace sj;
@@ -29,15 +30,29 @@ struct ResultHostCache {
     std::vector<int32_t> ids;
 };
 
-// host copy of a device-resident result, made on first host view
+// device-resident table downloaded as one host batch (rows in cell order)
+sj_status device_as_batch(sj_result r, sj_batch_host &B) {
+    const uint64_t n = r->n;
+    B.rowOffsets.resize(n + 1);
+    B.rowIds.resize(n);
+    B.ids.resize(r->total);
+    SJ_TRY_HIP(hipStreamSynchronize(r->ctx->stream));
+    SJ_TRY_HIP(hipMemcpy(B.rowOffsets.data(), r->d_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    SJ_TRY_HIP(hipMemcpy(B.rowIds.data(), r->d_rowIds.p, n * 4, hipMemcpyDeviceToHost));
+    if (r->total) SJ_TRY_HIP(hipMemcpy(B.ids.data(), r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost));
+    return SJ_OK;
+}
+
+// host copy in original point-ID order
 sj_status ensure_host_csr(sj_result r, std::vector<uint64_t> &off, std::vector<int32_t> &ids) {
     off.resize(r->n + 1);
     ids.resize(r->total);
     if (r->on_device) {
-        SJ_TRY_HIP(hipStreamSynchronize(r->ctx->stream));
-        SJ_TRY_HIP(hipMemcpy(off.data(), r->d_offsets.p, (r->n + 1) * 8, hipMemcpyDeviceToHost));
-        if (r->total) SJ_TRY_HIP(hipMemcpy(ids.data(), r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost));
-        return SJ_OK;
+        // materialise the host copy once, kept as batch 0
+        if (r->batches.empty()) {
+            r->batches.resize(1);
+            SJ_TRY(device_as_batch(r, r->batches[0]));
+        }
     }
     std::vector<uint64_t> cnt(r->n, 0);
     for (auto &B : r->batches)
@@ -261,8 +276,25 @@ SJ_API sj_status sj_result_device_csr(sj_result r, const uint64_t **d_offsets, c
         set_error("sj_result_device_csr: result was produced in host batches");
         return SJ_ERR_STATE;
     }
+    SJ_TRY(result_to_orig(r));
+    if (d_offsets) *d_offsets = r->d_offsetsOrig.p;
+    if (d_ids) *d_ids = r->d_idsOrig.p;
+    return SJ_OK;
+}
+
+SJ_API sj_status sj_result_device_table(sj_result r, const uint64_t **d_offsets, const int32_t **d_ids,
+                                        const uint32_t **d_rowIds) {
+    if (!r) {
+        set_error("sj_result_device_table: null result");
+        return SJ_ERR_INVALID;
+    }
+    if (!r->on_device) {
+        set_error("sj_result_device_table: result was produced in host batches");
+        return SJ_ERR_STATE;
+    }
     if (d_offsets) *d_offsets = r->d_offsets.p;
     if (d_ids) *d_ids = r->d_ids.p;
+    if (d_rowIds) *d_rowIds = r->d_rowIds.p;
     return SJ_OK;
 }
 
@@ -301,17 +333,9 @@ SJ_API sj_status sj_result_neighbor_table(sj_result r, sj_neighbor_lookup *table
         return SJ_ERR_INVALID;
     }
     for (uint64_t p = 0; p < r->n; p++) table[p] = {(int32_t)p, 0, -1, -1};
-    if (r->on_device) {
-        std::vector<uint64_t> off(r->n + 1);
-        SJ_TRY_HIP(hipStreamSynchronize(r->ctx->stream));
-        SJ_TRY_HIP(hipMemcpy(off.data(), r->d_offsets.p, (r->n + 1) * 8, hipMemcpyDeviceToHost));
-        if (r->total > 0x7fffffffull) {
-            set_error("neighbor table: more than 2^31-1 pairs in one batch (int32 indexmin)");
-            return SJ_ERR_OVERFLOW;
-        }
-        for (uint64_t p = 0; p < r->n; p++)
-            table[p] = {(int32_t)p, (int32_t)off[p], (int32_t)off[p + 1] - 1, 0};
-        return SJ_OK;
+    if (r->on_device && r->batches.empty()) {
+        r->batches.resize(1);
+        SJ_TRY(device_as_batch(r, r->batches[0]));
     }
     for (size_t b = 0; b < r->batches.size(); b++) {
         const auto &B = r->batches[b];
@@ -331,20 +355,9 @@ SJ_API sj_status sj_result_batch_data(sj_result r, uint64_t batch, const int32_t
         set_error("sj_result_batch_data: null argument");
         return SJ_ERR_INVALID;
     }
-    if (r->on_device) {
-        if (batch != 0) {
-            set_error("sj_result_batch_data: single batch result");
-            return SJ_ERR_INVALID;
-        }
-        // materialise the host copy once, kept as batch 0
-        if (r->batches.empty()) {
-            r->batches.resize(1);
-            std::vector<uint64_t> off;
-            SJ_TRY(ensure_host_csr(r, off, r->batches[0].ids));
-        }
-        *ids = r->batches[0].ids.data();
-        *count = r->batches[0].ids.size();
-        return SJ_OK;
+    if (r->on_device && r->batches.empty()) {
+        r->batches.resize(1);
+        SJ_TRY(device_as_batch(r, r->batches[0]));
     }
     if (batch >= r->batches.size()) {
         set_error("sj_result_batch_data: batch %llu of %zu", (unsigned long long)batch, r->batches.size());

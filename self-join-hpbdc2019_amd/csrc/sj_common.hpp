@@ -157,6 +157,7 @@ struct sj_grid_s {
     sj::DBuf<uint64_t> cellLinear;   // nNonEmpty
     sj::DBuf<uint32_t> cellCoord;    // nNonEmpty x k
     sj::DBuf<int32_t> denseCell;     // totalCells (dense lookup) or empty
+    sj::DBuf<uint32_t> posLB;        // totalCells+1: first sorted position with linear ID >= L
     sj::DBuf<float> sortedPts32;     // lazily built for the FP32 form
     double ms_build = 0.0;
 };
@@ -172,8 +173,11 @@ struct sj_result_s {
     uint64_t n = 0;
     uint64_t total = 0;
     bool on_device = false;
-    sj::DBuf<uint64_t> d_offsets;  // n+1, original ID order
-    sj::DBuf<int32_t> d_ids;       // total
+    sj::DBuf<uint64_t> d_offsets;      // n+1, rows in cell order
+    sj::DBuf<int32_t> d_ids;           // total
+    sj::DBuf<uint32_t> d_rowIds;       // original point ID of each row
+    sj::DBuf<uint64_t> d_offsetsOrig;  // n+1, original ID order (built on demand)
+    sj::DBuf<int32_t> d_idsOrig;
     std::vector<sj_batch_host> batches;
     sj_stats stats{};
 };

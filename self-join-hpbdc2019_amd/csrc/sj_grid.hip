@@ -187,6 +187,20 @@ __global__ __launch_bounds__(GB) void k_dense_scatter(const uint64_t *cellLinear
     dense[cellLinear[c]] = (int32_t)c;
 }
 
+__global__ __launch_bounds__(GB) void k_occupied(const int32_t *dense, uint64_t tc, uint32_t *occ) {
+    uint64_t L = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (L >= tc) return;
+    occ[L] = dense[L] >= 0 ? 1u : 0u;
+}
+
+// posLB[L] = cellStart[#non-empty cells with linear ID < L], L in [0, tc]
+__global__ __launch_bounds__(GB) void k_pos_lb(const uint32_t *lb, const uint32_t *cellStart, uint64_t tc,
+                                               uint32_t *posLB) {
+    uint64_t L = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    if (L > tc) return;
+    posLB[L] = cellStart[lb[L]];
+}
+
 static unsigned blocks_for(uint64_t n, unsigned b = GB) {
     uint64_t g = (n + b - 1) / b;
     return (unsigned)(g ? g : 1);
@@ -320,6 +334,17 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
         SJ_TRY(fill_u32(s, reinterpret_cast<uint32_t *>(g->denseCell.p), tc, 0xFFFFFFFFu));
         hipLaunchKernelGGL(k_dense_scatter, dim3(blocks_for(nne)), dim3(GB), 0, s, g->cellLinear.p, nne,
                            g->denseCell.p);
+        SJ_LAUNCH_CHECK();
+        // row-range lookup: first sorted position at or after each linear ID
+        DBuf<uint32_t> occ, lb;
+        SJ_TRY(occ.alloc(pool, tc));
+        SJ_TRY(lb.alloc(pool, tc + 1));
+        hipLaunchKernelGGL(k_occupied, dim3(blocks_for(tc)), dim3(GB), 0, s, g->denseCell.p, tc, occ.p);
+        SJ_LAUNCH_CHECK();
+        SJ_TRY(scan_exclusive_u32_to_u32(pool, s, occ.p, tc, lb.p));
+        SJ_TRY(g->posLB.alloc(pool, tc + 1));
+        hipLaunchKernelGGL(k_pos_lb, dim3(blocks_for(tc + 1)), dim3(GB), 0, s, lb.p, g->cellStart.p, tc,
+                           g->posLB.p);
         SJ_LAUNCH_CHECK();
     }
     return SJ_OK;

@@ -1,16 +1,25 @@
 // Cell-pair traversal of the grid join (reference kernel.cu:949-1538,
-// 2204-2531) producing the neighbour table as CSR, plus workload ordering
+// 2204-2531) producing the neighbour table, plus workload ordering
 // (kernel.cu:33-250, GPU.cu:1660-1734), the atomic work queue
 // (kernel.cu:2202-2236) and the batch estimator (kernel.cu:602-942).
 //
+// Work unit: a tile of 64 consecutive points in cell order, one wave.
+// When the tile's cells lie in one cell row (same coordinates in dims
+// 1..k-1), every neighbour row of the stencil is a contiguous range of
+// sorted positions: the wave stages that range in LDS once and each lane
+// walks its own three-cell sub-range (x-1, x, x+1) from LDS.  Tiles that
+// straddle a row fall back to a per-lane walk over neighbour cells.
+//
 // Two passes over the same traversal:
-//   count: per query point the hits it owns (own row) and, for the
-//          Unicomp forms, the mirrored hits it gives to the other point;
-//   fill:  own hits go to a lane-private cursor in the point's row, the
-//          mirrored hits are wave-aggregated: lanes hitting the same
-//          candidate take one atomic on that candidate's row cursor and
-//          place their IDs with a ballot popcount prefix.
-// Exact predicate: acc = sum fma((x_q - x_p)^2) (FP64, dimension order),
+//   count: hits each query point owns (own row) and, for the Unicomp
+//          forms, mirrored hits given to the candidate (LDS counters per
+//          staged candidate, one global atomic per candidate and chunk);
+//   fill:  own hits go to a lane-private cursor in the point's row; for
+//          mirrored hits each lane keeps a bitmask over the staged chunk,
+//          the wave reserves one range per candidate with one atomic, and
+//          lanes place their IDs through an LDS cursor.
+// Rows are laid out in cell order, so a tile writes one contiguous block.
+// Exact predicate: acc = sum fma((x_q - x_p)^2) in FP64, dimension order;
 // hit <=> acc <= T with T = max{x : sqrt_rn(x) <= eps}.
 #include <algorithm>
 #include <chrono>
@@ -21,10 +30,11 @@
 
 namespace sj {
 
-constexpr int JB = 256;          // traversal block size
-constexpr int DEV_MAXK = 8;      // indexed dims handled by the device stencil
+constexpr int JB = 256;         // traversal block size (4 waves)
+constexpr int WPB = JB / 64;    // waves per block
+constexpr int DEV_MAXK = 8;     // indexed dims handled by the device stencil
 enum { PASS_COUNT = 0, PASS_FILL = 1 };
-enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3 };
+enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
 
 struct JoinArgs {
     const double *pts;        // n x d, cell order
@@ -35,31 +45,37 @@ struct JoinArgs {
     const uint64_t *cellLinear;
     const uint32_t *cellCoord;
     const int32_t *dense;
+    const uint32_t *posLB;    // dense grids: first position with linear ID >= L
     uint64_t nne;
     uint32_t nCells[DEV_MAXK];
     int64_t mult[DEV_MAXK];
-    int d, k, form, pad0;
+    int d, k, form, useTiles;
     double T;
     float lo32, hi32, T32, pad1;
-    // query items: strided range, or the work queue when order != null
+    // query range (sorted positions) and tiles of 64 over it
     uint32_t qBegin, qEnd;
-    const uint32_t *order;
-    uint32_t nItems, chunk;
+    uint32_t nTiles, pad2;
+    const uint32_t *tileOrder;  // work queue order of tiles, or null
     uint32_t *qctr;
     // rows written by this launch (sorted positions)
     uint32_t wBegin, wEnd;
     // count pass
     uint32_t *ownCnt, *mirCnt;
     unsigned long long *stat;
-    // fill pass
-    const uint64_t *ownBase;
+    // fill pass: row r of the window starts at offS[r] - outBase
+    const uint64_t *offS;
+    uint64_t outBase;
     unsigned long long *mirCur;
-    const uint64_t *rowEnd;
     int32_t *ids;
 };
 
 __device__ __forceinline__ uint32_t popc_below64(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
@@ -70,6 +86,17 @@ __device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
         if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
     }
     return (lo < a.nne && a.cellLinear[lo] == lin) ? (int64_t)lo : -1;
+}
+
+// first sorted position whose cell has linear ID >= lin
+__device__ __forceinline__ uint32_t pos_lb(const JoinArgs &a, uint64_t lin) {
+    if (a.posLB) return a.posLB[lin];
+    uint64_t lo = 0, hi = a.nne;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
+    }
+    return a.cellStart[lo];
 }
 
 // Stencil claim rules: FULL every neighbour (kernel.cu:1076-1093); UNICOMP
@@ -93,7 +120,6 @@ __device__ __forceinline__ bool claims(int form, const uint32_t (&c)[K], const i
         }
         return odd;
     }
-    // B-UNICOMP
     uint32_t sc = 0;
 #pragma unroll
     for (int j = 0; j < K; j++) sc += c[j];
@@ -123,18 +149,9 @@ __device__ __forceinline__ bool claims(int form, const uint32_t (&c)[K], const i
     }
 }
 
-template <int D, int PREC>
-struct QueryPoint {
-    double x[D > 0 ? D : 1];
-    float y[D > 0 ? D : 1];
-};
-
-// FP64 canonical predicate, dimension order, explicit fma.
+// FP64 canonical predicate: dimension order, explicit fma.
 template <int D>
-__device__ __forceinline__ double dist2_fp64(const JoinArgs &a, const double *p, uint32_t t,
-                                             uint32_t s) {
-    const int d = D > 0 ? D : a.d;
-    const double *q = a.pts + (uint64_t)t * d;
+__device__ __forceinline__ double d2_fp64(const double *p, const double *q, int d) {
     double acc = 0.0;
     if constexpr (D > 0) {
 #pragma unroll
@@ -143,9 +160,8 @@ __device__ __forceinline__ double dist2_fp64(const JoinArgs &a, const double *p,
             acc = j == 0 ? df * df : fma(df, df, acc);
         }
     } else {
-        const double *pp = a.pts + (uint64_t)s * d;
         for (int j = 0; j < d; j++) {
-            const double df = q[j] - pp[j];
+            const double df = q[j] - p[j];
             acc = j == 0 ? df * df : fma(df, df, acc);
         }
     }
@@ -153,9 +169,7 @@ __device__ __forceinline__ double dist2_fp64(const JoinArgs &a, const double *p,
 }
 
 template <int D>
-__device__ __forceinline__ float dist2_fp32(const JoinArgs &a, const float *p, uint32_t t, uint32_t s) {
-    const int d = D > 0 ? D : a.d;
-    const float *q = a.pts32 + (uint64_t)t * d;
+__device__ __forceinline__ float d2_fp32(const float *p, const float *q, int d) {
     float acc = 0.0f;
     if constexpr (D > 0) {
 #pragma unroll
@@ -164,32 +178,59 @@ __device__ __forceinline__ float dist2_fp32(const JoinArgs &a, const float *p, u
             acc = j == 0 ? df * df : fmaf(df, df, acc);
         }
     } else {
-        const float *pp = a.pts32 + (uint64_t)s * d;
         for (int j = 0; j < d; j++) {
-            const float df = q[j] - pp[j];
+            const float df = q[j] - p[j];
             acc = j == 0 ? df * df : fmaf(df, df, acc);
         }
     }
     return acc;
 }
 
+// Decision for one candidate.  FP32 mode: outside the margin band the FP32
+// value decides, inside it the FP64 predicate does.
+template <int D, int PASS, int PREC>
+__device__ __forceinline__ bool decide(const JoinArgs &a, const double *p, const float *p32,
+                                       const float *q32, uint32_t t, uint64_t &band, uint64_t &mis) {
+    const int d = D > 0 ? D : a.d;
+    if constexpr (PREC == 1) {
+        const float s32 = d2_fp32<D>(p32, q32, d);
+        if (s32 < a.lo32) return true;
+        if (s32 > a.hi32) return false;
+        const bool hit = d2_fp64<D>(p, a.pts + (uint64_t)t * d, d) <= a.T;
+        if (PASS == PASS_COUNT) {
+            band++;
+            mis += (hit != (s32 <= a.T32));
+        }
+        return hit;
+    } else {
+        return d2_fp64<D>(p, a.pts + (uint64_t)t * d, d) <= a.T;
+    }
+}
+
+// ------------------------------------------------------- per-lane fallback
+// One query point, neighbour cells one at a time straight from memory.
 template <int K, int D, int PASS, int PREC>
 __device__ void process_point(const JoinArgs &a, uint32_t s) {
     constexpr int DR = D > 0 ? D : 1;
+    const int d = D > 0 ? D : a.d;
     const unsigned lane = __lane_id();
     const uint32_t cellA = a.cellOf[s];
     uint32_t c[K];
 #pragma unroll
     for (int j = 0; j < K; j++) c[j] = a.cellCoord[(uint64_t)cellA * K + j];
     const uint64_t linA = a.cellLinear[cellA];
-    double p[DR];
-    float p32[DR];
+    double pr[DR];
+    float p32r[DR];
+    const double *p = a.pts + (uint64_t)s * d;
+    const float *p32 = PREC == 1 ? a.pts32 + (uint64_t)s * d : nullptr;
     if constexpr (D > 0) {
 #pragma unroll
-        for (int j = 0; j < D; j++) p[j] = a.pts[(uint64_t)s * D + j];
+        for (int j = 0; j < D; j++) pr[j] = p[j];
+        p = pr;
         if constexpr (PREC == 1) {
 #pragma unroll
-            for (int j = 0; j < D; j++) p32[j] = a.pts32[(uint64_t)s * D + j];
+            for (int j = 0; j < D; j++) p32r[j] = p32[j];
+            p32 = p32r;
         }
     }
     const bool ownW = s >= a.wBegin && s < a.wEnd;
@@ -197,7 +238,7 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
     uint32_t own = 0;
     uint64_t ownPos = 0, ownLim = 0;
     if (PASS == PASS_FILL && ownW) {
-        ownPos = a.ownBase[s];
+        ownPos = a.offS[s] - a.outBase;
         ownLim = ownPos + a.ownCnt[s];
     }
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
@@ -207,32 +248,15 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
         const uint32_t t0 = a.cellStart[b], t1 = a.cellStart[b + 1];
         if (PASS == PASS_COUNT) cand += t1 - t0;
         for (uint32_t t = t0; t < t1; ++t) {
-            bool hit;
-            if constexpr (PREC == 1) {
-                const float s32 = dist2_fp32<D>(a, p32, t, s);
-                if (s32 < a.lo32) {
-                    hit = true;
-                } else if (s32 > a.hi32) {
-                    hit = false;
-                } else {
-                    hit = dist2_fp64<D>(a, p, t, s) <= a.T;
-                    if (PASS == PASS_COUNT) {
-                        band++;
-                        mis += (hit != (s32 <= a.T32));
-                    }
-                }
-            } else {
-                hit = dist2_fp64<D>(a, p, t, s) <= a.T;
-            }
+            const float *q32 = PREC == 1 ? a.pts32 + (uint64_t)t * d : nullptr;
+            const bool hit = decide<D, PASS, PREC>(a, p, p32, q32, t, band, mis);
             if (hit && ownW) {
-                if (PASS == PASS_COUNT) {
-                    own++;
-                } else {
+                if (PASS == PASS_FILL) {
                     const uint64_t w = ownPos + own;
                     if (w < ownLim) a.ids[w] = (int32_t)a.origId[t];
                     else err++;
-                    own++;
                 }
+                own++;
             }
             if (mirror) {
                 const bool mh = hit && t >= a.wBegin && t < a.wEnd;
@@ -253,7 +277,7 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
                             uint64_t w;
                             if (t == tl) w = base + popc_below64(same);
                             else w = atomicAdd(&a.mirCur[t], 1ull);
-                            if (w < a.rowEnd[t]) a.ids[w] = myId;
+                            if (w < a.offS[t + 1] - a.outBase) a.ids[w] = myId;
                             else err++;
                         }
                     }
@@ -262,10 +286,7 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
         }
     };
 
-    // origin cell: every ordered pair, self included (kernel.cu:445-496)
     visit(cellA, false);
-
-    // neighbour cells by odometer over {-1,0,1}^K
     int o[K];
 #pragma unroll
     for (int j = 0; j < K; j++) o[j] = -1;
@@ -313,22 +334,281 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
     }
 }
 
+// ------------------------------------------------------------ tile path
+template <int D, int PREC>
+struct TileCfg {
+    static constexpr int DD = D > 0 ? D : 1;
+    static constexpr int CH = (PREC == 1 || D <= 4) ? 256 : 128;  // staged candidates per chunk
+    static constexpr int MW = CH / 64;                              // mask words per lane
+};
+
+template <int D, int PREC>
+struct TileLds {
+    using C = TileCfg<D, PREC>;
+    double x64[PREC == 1 ? 1 : C::CH][C::DD];
+    float x32[PREC == 1 ? C::CH : 1][C::DD];
+    int32_t id[C::CH];
+    uint32_t cnt[C::CH];
+    unsigned long long base[C::CH];
+};
+
+template <int K, int D, int PASS, int PREC>
+__device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> &L) {
+    using Cfg = TileCfg<D, PREC>;
+    constexpr int CH = Cfg::CH;
+    constexpr int MW = Cfg::MW;
+    constexpr int DD = Cfg::DD;
+    const unsigned lane = __lane_id();
+    const uint64_t s64 = (uint64_t)a.qBegin + (uint64_t)tile * 64 + lane;
+    const bool valid = s64 < a.qEnd;
+    const uint32_t s = valid ? (uint32_t)s64 : 0u;
+    const uint64_t vmask = __ballot(valid);
+    if (!vmask) return;
+    const int firstL = __builtin_ctzll(vmask);
+    const int lastL = 63 - __builtin_clzll(vmask);
+
+    // lanes past the end take the last point's cell so that row-level
+    // control flow stays uniform across the wave; they never emit
+    const uint32_t sEff = valid ? s : (uint32_t)(a.qBegin + (uint64_t)tile * 64 + lastL);
+    uint32_t c[K];
+    const uint32_t cellA = a.cellOf[sEff];
+#pragma unroll
+    for (int j = 0; j < K; j++) c[j] = a.cellCoord[(uint64_t)cellA * K + j];
+    const uint64_t linA = a.cellLinear[cellA];
+    const uint64_t rowBase = linA - c[0];
+    const uint64_t rb0 = __shfl(rowBase, firstL, 64);
+    const uint64_t same = __ballot(valid && rowBase == rb0);
+    if (same != vmask) {
+        // the tile crosses a cell row: per-lane walk
+        if (valid) process_point<K, D, PASS, PREC>(a, s);
+        if (PASS == PASS_COUNT && lane == (unsigned)firstL) atomicAdd(&a.stat[STAT_FALLBACK], 1ull);
+        return;
+    }
+    const uint32_t xa = __shfl(c[0], firstL, 64);
+    const uint32_t xb = __shfl(c[0], lastL, 64);
+    const uint32_t nC0 = a.nCells[0];
+    const uint32_t x = c[0];
+
+    // query point in registers
+    double p[DD];
+    float p32[DD];
+    const int d = D > 0 ? D : a.d;
+    if constexpr (D > 0) {
+        if (valid) {
+#pragma unroll
+            for (int j = 0; j < D; j++) p[j] = a.pts[(uint64_t)s * D + j];
+            if constexpr (PREC == 1) {
+#pragma unroll
+                for (int j = 0; j < D; j++) p32[j] = a.pts32[(uint64_t)s * D + j];
+            }
+        }
+    }
+    const bool ownW = valid && s >= a.wBegin && s < a.wEnd;
+    const int32_t myId = valid ? (int32_t)a.origId[s] : 0;
+    uint32_t own = 0;
+    uint64_t ownPos = 0, ownLim = 0;
+    if (PASS == PASS_FILL && ownW) {
+        ownPos = a.offS[s] - a.outBase;
+        ownLim = ownPos + a.ownCnt[s];
+    }
+    uint64_t cand = 0, band = 0, mis = 0, err = 0;
+    const bool anyMirror = a.form != SJ_FORM_FULL;
+
+    // neighbour rows: odometer over dims 1..K-1
+    int o[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) o[j] = -1;
+    o[0] = 0;
+    int nrow = 1;
+#pragma unroll
+    for (int j = 1; j < K; j++) nrow *= 3;
+    if constexpr (K == 1) o[0] = 0;
+    for (int it = 0; it < nrow; ++it) {
+        bool inb = true, originRow = true;
+        int64_t dl = 0;
+#pragma unroll
+        for (int j = 1; j < K; j++) {
+            originRow = originRow && o[j] == 0;
+            const int64_t v = (int64_t)c[j] + o[j];
+            inb = inb && v >= 0 && v < (int64_t)a.nCells[j];
+            dl += (int64_t)o[j] * a.mult[j];
+        }
+        // inb is uniform: every lane shares the row coordinates
+        if (inb) {
+            const uint64_t rb = (uint64_t)((int64_t)rowBase + dl);
+            // claimed dim-0 offsets of this lane's cell in this row
+            bool useL = false, useC = false, useR = false;
+            if (valid) {
+                int oo[K];
+#pragma unroll
+                for (int j = 0; j < K; j++) oo[j] = o[j];
+                oo[0] = -1;
+                useL = x >= 1 && claims<K>(a.form, c, oo, linA, rb + x - 1);
+                oo[0] = 1;
+                useR = x + 1 < nC0 && claims<K>(a.form, c, oo, linA, rb + x + 1);
+                oo[0] = 0;
+                useC = originRow || claims<K>(a.form, c, oo, linA, rb + x);
+            }
+            if (__ballot(useL || useC || useR)) {
+                const uint32_t ux0 = xa >= 1 ? xa - 1 : 0u;
+                const uint32_t ux1 = xb + 1 < nC0 ? xb + 1 : nC0 - 1;
+                const uint32_t u0 = pos_lb(a, rb + ux0);
+                const uint32_t u1 = pos_lb(a, rb + ux1 + 1);
+                uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+                if (valid) {
+                    e1 = pos_lb(a, rb + x);
+                    e2 = pos_lb(a, rb + x + 1);
+                    e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
+                    e3 = x + 1 < nC0 ? pos_lb(a, rb + x + 2) : e2;
+                }
+                const bool mirL = anyMirror, mirR = anyMirror, mirC = anyMirror && !originRow;
+                if (PASS == PASS_COUNT && valid)
+                    cand += (useL ? e1 - e0 : 0) + (useC ? e2 - e1 : 0) + (useR ? e3 - e2 : 0);
+                for (uint32_t c0 = u0; c0 < u1; c0 += CH) {
+                    const uint32_t m = min((uint32_t)CH, u1 - c0);
+                    // stage the chunk
+                    for (uint32_t j = lane; j < m; j += 64) {
+                        const uint64_t t = (uint64_t)c0 + j;
+                        if constexpr (PREC == 1) {
+#pragma unroll
+                            for (int q = 0; q < DD; q++) L.x32[j][q] = a.pts32[t * DD + q];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
+                        }
+                        L.id[j] = (int32_t)a.origId[t];
+                        L.cnt[j] = 0u;
+                    }
+                    wave_lds_sync();
+                    uint64_t mask[MW];
+#pragma unroll
+                    for (int w = 0; w < MW; w++) mask[w] = 0ull;
+                    bool anyMir = false;
+                    // walk the claimed sub-ranges inside this chunk
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        const bool use = r == 0 ? useL : (r == 1 ? useC : useR);
+                        const bool mir = r == 0 ? mirL : (r == 1 ? mirC : mirR);
+                        const uint32_t rs = r == 0 ? e0 : (r == 1 ? e1 : e2);
+                        const uint32_t re = r == 0 ? e1 : (r == 1 ? e2 : e3);
+                        if (!use) continue;
+                        const uint32_t lo = max(rs, c0), hi = min(re, c0 + m);
+                        for (uint32_t t = lo; t < hi; ++t) {
+                            const uint32_t j = t - c0;
+                            bool hit;
+                            if constexpr (PREC == 1) {
+                                hit = decide<D, PASS, PREC>(a, p, p32, &L.x32[j][0], t, band, mis);
+                            } else {
+                                hit = d2_fp64<D>(p, &L.x64[j][0], d) <= a.T;
+                            }
+                            if (!hit) continue;
+                            if (ownW) {
+                                if (PASS == PASS_FILL) {
+                                    const uint64_t w = ownPos + own;
+                                    if (w < ownLim) a.ids[w] = L.id[j];
+                                    else err++;
+                                }
+                                own++;
+                            }
+                            if (mir && t >= a.wBegin && t < a.wEnd) {
+                                atomicAdd(&L.cnt[j], 1u);
+                                if (PASS == PASS_FILL) {
+#pragma unroll
+                                    for (int w = 0; w < MW; w++)
+                                        if ((int)(j >> 6) == w) mask[w] |= 1ull << (j & 63);
+                                    anyMir = true;
+                                }
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                    if (anyMirror) {
+                        if (PASS == PASS_COUNT) {
+                            for (uint32_t j = lane; j < m; j += 64) {
+                                const uint32_t v = L.cnt[j];
+                                if (v) atomicAdd(&a.mirCnt[c0 + j], v);
+                            }
+                        } else {
+                            // reserve one range per candidate, then place
+                            for (uint32_t j = lane; j < m; j += 64) {
+                                const uint32_t v = L.cnt[j];
+                                if (v) L.base[j] = atomicAdd(&a.mirCur[c0 + j], (unsigned long long)v);
+                                L.cnt[j] = 0u;
+                            }
+                            wave_lds_sync();
+                            if (anyMir) {
+#pragma unroll
+                                for (int w = 0; w < MW; w++) {
+                                    uint64_t mm = mask[w];
+                                    while (mm) {
+                                        const uint32_t j = (uint32_t)(w * 64 + __builtin_ctzll(mm));
+                                        mm &= mm - 1;
+                                        const uint32_t k2 = atomicAdd(&L.cnt[j], 1u);
+                                        const uint64_t wpos = L.base[j] + k2;
+                                        if (wpos < a.offS[c0 + j + 1] - a.outBase) a.ids[wpos] = myId;
+                                        else err++;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                }
+            }
+        }
+        bool carry = true;
+#pragma unroll
+        for (int j = 1; j < K; j++) {
+            if (carry) {
+                if (o[j] == 1) {
+                    o[j] = -1;
+                } else {
+                    o[j]++;
+                    carry = false;
+                }
+            }
+        }
+    }
+    if (PASS == PASS_COUNT) {
+        if (ownW) a.ownCnt[s] = own;
+        if (cand) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)cand);
+        if (band) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)band);
+        if (mis) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)mis);
+    } else {
+        if (ownW && own != (ownLim - ownPos)) err++;
+        if (err) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)err);
+    }
+}
+
 template <int K, int D, int PASS, int PREC>
 __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
-    if (a.order) {
+    __shared__ TileLds<D, PREC> lds[WPB];
+    const unsigned wave = threadIdx.x >> 6;
+    TileLds<D, PREC> &L = lds[wave];
+    if (a.tileOrder) {
         const unsigned lane = __lane_id();
         for (;;) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(a.qctr, a.chunk);
-            base = __shfl(base, 0, 64);
-            if (base >= a.nItems) break;
-            const uint32_t end = min(base + a.chunk, a.nItems);
-            for (uint32_t it = base + lane; it < end; it += 64)
-                process_point<K, D, PASS, PREC>(a, a.order[it]);
+            uint32_t idx = 0;
+            if (lane == 0) idx = atomicAdd(a.qctr, 1u);
+            idx = __shfl(idx, 0, 64);
+            if (idx >= a.nTiles) break;
+            const uint32_t tile = a.tileOrder[idx];
+            if (D > 0 && a.useTiles) {
+                process_tile<K, D, PASS, PREC>(a, tile, L);
+            } else {
+                const uint64_t s = (uint64_t)a.qBegin + (uint64_t)tile * 64 + lane;
+                if (s < a.qEnd) process_point<K, D, PASS, PREC>(a, (uint32_t)s);
+            }
         }
     } else {
-        const uint64_t s = (uint64_t)a.qBegin + (uint64_t)blockIdx.x * JB + threadIdx.x;
-        if (s < a.qEnd) process_point<K, D, PASS, PREC>(a, (uint32_t)s);
+        const uint32_t tile = blockIdx.x * WPB + wave;
+        if (tile >= a.nTiles) return;
+        if (D > 0 && a.useTiles) {
+            process_tile<K, D, PASS, PREC>(a, tile, L);
+        } else {
+            const uint64_t s = (uint64_t)a.qBegin + (uint64_t)tile * 64 + __lane_id();
+            if (s < a.qEnd) process_point<K, D, PASS, PREC>(a, (uint32_t)s);
+        }
     }
 }
 
@@ -391,59 +671,33 @@ __global__ void k_desc_keys(const uint32_t *w, uint64_t n, uint32_t *key, uint32
     val[i] = (uint32_t)i;
 }
 
-__global__ void k_order_sizes(const uint32_t *orderCell, const uint32_t *cellStart, uint64_t nne,
-                              uint32_t *size, uint32_t *rank) {
-    uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (i >= nne) return;
-    const uint32_t c = orderCell[i];
-    size[i] = cellStart[c + 1] - cellStart[c];
-    rank[c] = (uint32_t)i;
-}
-
-__global__ void k_queue_points(const uint32_t *cellOf, const uint32_t *cellStart, const uint32_t *rank,
-                               const uint32_t *pos, uint64_t n, uint32_t *queue) {
-    uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t c = cellOf[s];
-    queue[pos[rank[c]] + ((uint32_t)s - cellStart[c])] = (uint32_t)s;
+// tile cost: candidates of its points (cell workload per point)
+__global__ void k_tile_cost(const uint32_t *w, const uint32_t *cellOf, uint32_t q0, uint32_t q1,
+                            uint32_t nTiles, uint32_t *key, uint32_t *val) {
+    const uint64_t s = (uint64_t)q0 + (uint64_t)blockIdx.x * JB + threadIdx.x;
+    const uint32_t tile = (blockIdx.x * JB + threadIdx.x) >> 6;
+    uint32_t v = s < q1 ? w[cellOf[s]] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0 && tile < nTiles) {
+        key[tile] = 0xFFFFFFFFu - v;
+        val[tile] = tile;
+    }
 }
 
 // ----------------------------------------------------- small row kernels
-__global__ void k_totals_to_orig(const uint32_t *own, const uint32_t *mir, const uint32_t *origId,
-                                 uint64_t n, uint32_t *cntOrig) {
-    uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (s >= n) return;
-    cntOrig[origId[s]] = own[s] + mir[s];
-}
-
 __global__ void k_totals_sorted(const uint32_t *own, const uint32_t *mir, uint64_t n, uint32_t *tot) {
     uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
     if (s >= n) return;
     tot[s] = own[s] + mir[s];
 }
 
-// whole result: rows in original ID order
-__global__ void k_row_bases_orig(const uint64_t *offsets, const uint32_t *origId, const uint32_t *own,
-                                 uint64_t n, uint64_t *ownBase, unsigned long long *mirCur,
-                                 uint64_t *rowEnd) {
-    uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t id = origId[s];
-    const uint64_t b = offsets[id];
-    ownBase[s] = b;
-    mirCur[s] = b + own[s];
-    rowEnd[s] = offsets[id + 1];
-}
-
-// batch: rows [w0, w1) in sorted order, relative to offS[w0]
-__global__ void k_row_bases_window(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
-                                   uint64_t *ownBase, unsigned long long *mirCur, uint64_t *rowEnd) {
+// mirrored-hit cursors of rows [w0, w1): start right after the own hits
+__global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
+                              uint64_t outBase, unsigned long long *mirCur) {
     uint64_t s = (uint64_t)w0 + (uint64_t)blockIdx.x * JB + threadIdx.x;
     if (s >= w1) return;
-    const uint64_t b = offS[s] - offS[w0];
-    ownBase[s] = b;
-    mirCur[s] = b + own[s];
-    rowEnd[s] = offS[s + 1] - offS[w0];
+    mirCur[s] = offS[s] - outBase + own[s];
 }
 
 __global__ void k_to_fp32(const double *pts, uint64_t n, int d, int k, const GridParams *gp, float *out,
@@ -472,6 +726,14 @@ __global__ void k_sample_positions(const uint32_t *sortedIds, uint64_t n, uint64
     }
 }
 
+// estimator: every stride-th original point, full stencil, counted per point
+template <int K, int D>
+__global__ __launch_bounds__(JB) void k_sample_count(JoinArgs a, const uint32_t *list, uint64_t m) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (i >= m) return;
+    process_point<K, D, PASS_COUNT, 0>(a, list[i]);
+}
+
 __global__ void k_sum_gather(const uint32_t *v, const uint32_t *idx, uint64_t m, unsigned long long *out) {
     uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
     unsigned long long x = i < m ? v[idx[i]] : 0ull;
@@ -487,6 +749,7 @@ static unsigned nblk(uint64_t n) {
 }
 
 using JoinKernel = void (*)(JoinArgs);
+using SampleKernel = void (*)(JoinArgs, const uint32_t *, uint64_t);
 
 template <int K, int D>
 static JoinKernel pick_pass(int pass, int prec) {
@@ -510,6 +773,25 @@ static JoinKernel pick_kernel(int k, int d, int pass, int prec) {
     case 6: return pick_d<6>(d, pass, prec);
     case 7: return pick_d<7>(d, pass, prec);
     case 8: return pick_d<8>(d, pass, prec);
+    }
+    return nullptr;
+}
+
+template <int K>
+static SampleKernel pick_sample_d(int d) {
+    return d == K ? k_sample_count<K, K> : k_sample_count<K, 0>;
+}
+
+static SampleKernel pick_sample(int k, int d) {
+    switch (k) {
+    case 1: return pick_sample_d<1>(d);
+    case 2: return pick_sample_d<2>(d);
+    case 3: return pick_sample_d<3>(d);
+    case 4: return pick_sample_d<4>(d);
+    case 5: return pick_sample_d<5>(d);
+    case 6: return pick_sample_d<6>(d);
+    case 7: return pick_sample_d<7>(d);
+    case 8: return pick_sample_d<8>(d);
     }
     return nullptr;
 }
@@ -588,6 +870,7 @@ static Fp32Margin fp32_margin(double T, double R, int D) {
 struct EvTimer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
+    bool used = false;
     explicit EvTimer(hipStream_t st) : s(st) {
         (void)hipEventCreate(&a);
         (void)hipEventCreate(&b);
@@ -596,9 +879,13 @@ struct EvTimer {
         if (a) (void)hipEventDestroy(a);
         if (b) (void)hipEventDestroy(b);
     }
-    void start() { (void)hipEventRecord(a, s); }
+    void start() {
+        (void)hipEventRecord(a, s);
+        used = true;
+    }
     void stop() { (void)hipEventRecord(b, s); }
     double ms() {
+        if (!used) return 0.0;
         float f = 0;
         (void)hipEventSynchronize(b);
         (void)hipEventElapsedTime(&f, a, b);
@@ -616,6 +903,7 @@ static JoinArgs base_args(sj_grid_s *g, int form, double T) {
     a.cellLinear = g->cellLinear.p;
     a.cellCoord = g->cellCoord.p;
     a.dense = g->dense ? g->denseCell.p : nullptr;
+    a.posLB = g->dense ? g->posLB.p : nullptr;
     a.nne = g->nNonEmpty;
     for (int j = 0; j < DEV_MAXK; j++) {
         a.nCells[j] = j < g->k ? g->gp.nCells[j] : 1;
@@ -624,50 +912,43 @@ static JoinArgs base_args(sj_grid_s *g, int form, double T) {
     a.d = g->d;
     a.k = g->k;
     a.form = form;
+    a.useTiles = 1;
     a.T = T;
     a.qBegin = 0;
     a.qEnd = (uint32_t)g->n;
+    a.nTiles = (uint32_t)((g->n + 63) / 64);
     a.wBegin = 0;
     a.wEnd = (uint32_t)g->n;
     return a;
 }
 
-// Cells in descending workload, flattened to a queue of sorted positions.
-static sj_status build_queue(sj_grid_s *g, int lid, DBuf<uint32_t> &queue, DBuf<uint32_t> *orderOut,
-                             DBuf<uint32_t> *workOut) {
+static sj_status cell_workload(sj_grid_s *g, int lid, DBuf<uint32_t> &w) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
-    const uint64_t nne = g->nNonEmpty, n = g->n;
+    const uint64_t nne = g->nNonEmpty;
     JoinArgs a = base_args(g, SJ_FORM_FULL, 0.0);
-    DBuf<uint32_t> w, key, key2, val, val2;
     SJ_TRY(w.alloc(pool, nne));
-    SJ_TRY(key.alloc(pool, nne));
-    SJ_TRY(key2.alloc(pool, nne));
-    SJ_TRY(val.alloc(pool, nne));
-    SJ_TRY(val2.alloc(pool, nne));
     hipLaunchKernelGGL(pick_workload(g->k), dim3(nblk(nne)), dim3(JB), 0, s, a, lid, w.p);
     SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_desc_keys, dim3(nblk(nne)), dim3(JB), 0, s, w.p, nne, key.p, val.p);
+    return SJ_OK;
+}
+
+// Work queue: 64-point tiles in descending candidate count.
+static sj_status build_tile_queue(sj_grid_s *g, int lid, uint32_t nTiles, DBuf<uint32_t> &order) {
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    DBuf<uint32_t> w, key, key2, val2;
+    SJ_TRY(cell_workload(g, lid, w));
+    SJ_TRY(key.alloc(pool, nTiles));
+    SJ_TRY(key2.alloc(pool, nTiles));
+    SJ_TRY(order.alloc(pool, nTiles));
+    SJ_TRY(val2.alloc(pool, nTiles));
+    hipLaunchKernelGGL(k_tile_cost, dim3(nblk((uint64_t)nTiles * 64)), dim3(JB), 0, s, w.p, g->cellOf.p, 0u,
+                       (uint32_t)g->n, nTiles, key.p, order.p);
     SJ_LAUNCH_CHECK();
-    uint32_t *kp = key.p, *ka = key2.p, *vp = val.p, *va = val2.p;
-    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nne, 32));
-    DBuf<uint32_t> size, rank, pos;
-    SJ_TRY(size.alloc(pool, nne));
-    SJ_TRY(rank.alloc(pool, nne));
-    SJ_TRY(pos.alloc(pool, nne + 1));
-    hipLaunchKernelGGL(k_order_sizes, dim3(nblk(nne)), dim3(JB), 0, s, vp, g->cellStart.p, nne, size.p,
-                       rank.p);
-    SJ_LAUNCH_CHECK();
-    SJ_TRY(scan_exclusive_u32_to_u32(pool, s, size.p, nne, pos.p));
-    SJ_TRY(queue.alloc(pool, n));
-    hipLaunchKernelGGL(k_queue_points, dim3(nblk(n)), dim3(JB), 0, s, g->cellOf.p, g->cellStart.p, rank.p,
-                       pos.p, n, queue.p);
-    SJ_LAUNCH_CHECK();
-    if (orderOut) {
-        SJ_TRY(orderOut->alloc(pool, nne));
-        SJ_TRY_HIP(hipMemcpyAsync(orderOut->p, vp, nne * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    }
-    if (workOut) *workOut = std::move(w);
+    uint32_t *kp = key.p, *ka = key2.p, *vp = order.p, *va = val2.p;
+    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nTiles, 32));
+    if (vp != order.p) std::swap(order, val2);
     return SJ_OK;
 }
 
@@ -676,18 +957,29 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
         set_error("workload: k=%d above the device stencil limit %d", g->k, DEV_MAXK);
         return SJ_ERR_UNSUPPORTED;
     }
-    DBuf<uint32_t> queue, ord, w;
-    SJ_TRY(build_queue(g, lid, queue, &ord, &w));
-    std::vector<uint32_t> hw(g->nNonEmpty);
-    SJ_TRY_HIP(hipStreamSynchronize(g->ctx->stream));
-    SJ_TRY_HIP(hipMemcpy(hw.data(), w.p, hw.size() * 4, hipMemcpyDeviceToHost));
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    const uint64_t nne = g->nNonEmpty;
+    DBuf<uint32_t> w, key, key2, val, val2;
+    SJ_TRY(cell_workload(g, lid, w));
+    SJ_TRY(key.alloc(pool, nne));
+    SJ_TRY(key2.alloc(pool, nne));
+    SJ_TRY(val.alloc(pool, nne));
+    SJ_TRY(val2.alloc(pool, nne));
+    hipLaunchKernelGGL(k_desc_keys, dim3(nblk(nne)), dim3(JB), 0, s, w.p, nne, key.p, val.p);
+    SJ_LAUNCH_CHECK();
+    uint32_t *kp = key.p, *ka = key2.p, *vp = val.p, *va = val2.p;
+    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nne, 32));
+    std::vector<uint32_t> hw(nne);
+    SJ_TRY_HIP(hipStreamSynchronize(s));
+    SJ_TRY_HIP(hipMemcpy(hw.data(), w.p, nne * 4, hipMemcpyDeviceToHost));
     if (nb)
-        for (uint64_t i = 0; i < hw.size(); i++) nb[i] = hw[i];
-    if (order) SJ_TRY_HIP(hipMemcpy(order, ord.p, g->nNonEmpty * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < nne; i++) nb[i] = hw[i];
+    if (order) SJ_TRY_HIP(hipMemcpy(order, vp, nne * 4, hipMemcpyDeviceToHost));
     return SJ_OK;
 }
 
-static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, uint32_t nItems) {
+static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec) {
     hipStream_t s = g->ctx->stream;
     JoinKernel kfn = pick_kernel(g->k, g->d, pass, prec);
     if (!kfn) {
@@ -695,11 +987,11 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
         return SJ_ERR_UNSUPPORTED;
     }
     unsigned blocks;
-    if (a.order) {
-        blocks = (unsigned)std::min<uint64_t>((uint64_t)g->ctx->cu_count * 8, nblk(nItems));
-    } else {
-        blocks = nblk((uint64_t)a.qEnd - a.qBegin);
-    }
+    if (a.tileOrder)
+        blocks = (unsigned)std::min<uint64_t>((uint64_t)g->ctx->cu_count * 8, (a.nTiles + WPB - 1) / WPB);
+    else
+        blocks = (a.nTiles + WPB - 1) / WPB;
+    if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(kfn, dim3(blocks), dim3(JB), 0, s, a);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
@@ -710,16 +1002,19 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t n = g->n;
+    if (g->k > DEV_MAXK) {
+        set_error("estimator: k=%d above the device stencil limit %d", g->k, DEV_MAXK);
+        return SJ_ERR_UNSUPPORTED;
+    }
     if (stride == 0) stride = 1;
     const uint64_t nsamp = (uint64_t)((double)n * (1.0 / (double)stride));
     *count = 0;
     if (nsamp == 0) return SJ_OK;
-    DBuf<uint32_t> posOf, list, own, qctr;
+    DBuf<uint32_t> posOf, list, own;
     DBuf<unsigned long long> stat;
     SJ_TRY(posOf.alloc(pool, n));
     SJ_TRY(list.alloc(pool, nsamp));
     SJ_TRY(own.alloc(pool, n));
-    SJ_TRY(qctr.alloc(pool, 1));
     SJ_TRY(stat.alloc(pool, 8));
     hipLaunchKernelGGL(k_sample_positions, dim3(nblk(n)), dim3(JB), 0, s, g->sortedIds.p, n, stride, nsamp,
                        posOf.p, list.p, 0);
@@ -727,34 +1022,23 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
     hipLaunchKernelGGL(k_sample_positions, dim3(nblk(nsamp)), dim3(JB), 0, s, g->sortedIds.p, n, stride,
                        nsamp, posOf.p, list.p, 1);
     SJ_LAUNCH_CHECK();
-    SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p), 8, 0ull));
     JoinArgs a = base_args(g, SJ_FORM_FULL, eps_threshold(g->eps));
-    a.order = list.p;
-    a.nItems = (uint32_t)nsamp;
-    a.chunk = 64;
-    a.qctr = qctr.p;
     a.ownCnt = own.p;
     a.mirCnt = own.p;  // unused by FULL
     a.stat = stat.p;
-    SJ_TRY(launch_join(g, a, PASS_COUNT, 0, (uint32_t)nsamp));
-    SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + 4, 1, 0ull));
-    hipLaunchKernelGGL(k_sum_gather, dim3(nblk(nsamp)), dim3(JB), 0, s, own.p, list.p, nsamp, stat.p + 4);
+    hipLaunchKernelGGL(pick_sample(g->k, g->d), dim3(nblk(nsamp)), dim3(JB), 0, s, a, list.p, nsamp);
     SJ_LAUNCH_CHECK();
-    unsigned long long h = 0;
-    SJ_TRY_HIP(hipMemcpyAsync(&h, stat.p + 4, 8, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_sum_gather, dim3(nblk(nsamp)), dim3(JB), 0, s, own.p, list.p, nsamp, stat.p + 5);
+    SJ_LAUNCH_CHECK();
+    uint64_t *pin = g->ctx->pinned;
+    SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + 5, 8, hipMemcpyDeviceToHost, s));
     SJ_TRY_HIP(hipStreamSynchronize(s));
-    *count = h;
+    *count = pin[0];
     return SJ_OK;
 }
 
-struct JoinPlan {
-    int form, prec, sched, sort_rows;
-    uint64_t capacity;
-    uint32_t chunk;
-};
-
-sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n);
+sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base);
 
 sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     const auto t_wall0 = std::chrono::steady_clock::now();
@@ -784,6 +1068,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     memset(&r->stats, 0, sizeof r->stats);
     const double T = eps_threshold(g->eps);
     JoinArgs a = base_args(g, P->form, T);
+    if (P->cells_per_tile < 0) a.useTiles = 0;  // per-point walk only (diagnostics)
     EvTimer tw(s), tc(s), ts(s), tf(s), tr(s);
 
     // precision form: FP32 copy relative to the grid origin + margin
@@ -796,9 +1081,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n * (uint64_t)g->d)), dim3(JB), 0, s, g->sortedPts.p, n, g->d,
                            g->k, g->d_gp.p, g->sortedPts32.p, rbits.p);
         SJ_LAUNCH_CHECK();
-        uint32_t hb = 0;
-        SJ_TRY_HIP(hipMemcpyAsync(&hb, rbits.p, 4, hipMemcpyDeviceToHost, s));
+        uint64_t *pin = ctx->pinned;
+        SJ_TRY_HIP(hipMemcpyAsync(pin, rbits.p, 4, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipStreamSynchronize(s));
+        const uint32_t hb = (uint32_t)(pin[0] & 0xffffffffu);
         float rf;
         memcpy(&rf, &hb, 4);
         // the FP32 rounding of |v| may be below |v|: widen by one FP32 ulp
@@ -814,16 +1100,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         }
     }
 
-    // scheduling
-    DBuf<uint32_t> queue, qctr;
+    // scheduling: work queue of tiles in descending workload
+    DBuf<uint32_t> tileOrder, qctr;
     if (P->schedule == SJ_SCHED_WORKQUEUE) {
         tw.start();
-        SJ_TRY(build_queue(g, P->form == SJ_FORM_LID ? 1 : 0, queue, nullptr, nullptr));
+        SJ_TRY(build_tile_queue(g, P->form == SJ_FORM_LID ? 1 : 0, a.nTiles, tileOrder));
         tw.stop();
         SJ_TRY(qctr.alloc(pool, 1));
-        a.order = queue.p;
-        a.nItems = (uint32_t)n;
-        a.chunk = P->queue_chunk > 0 ? (uint32_t)P->queue_chunk : 64u;
+        a.tileOrder = tileOrder.p;
         a.qctr = qctr.p;
     }
 
@@ -835,27 +1119,26 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(stat.alloc(pool, 8));
     SJ_TRY(fill_u32(s, mir.p, n, 0u));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p), 8, 0ull));
-    if (a.order) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
+    if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
     a.stat = stat.p;
     tc.start();
-    SJ_TRY(launch_join(g, a, PASS_COUNT, prec, (uint32_t)n));
+    SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();
 
-    // offsets in original ID order (the neighbour table product)
+    // row offsets in cell order
     ts.start();
-    DBuf<uint32_t> cntOrig;
-    SJ_TRY(cntOrig.alloc(pool, n));
+    DBuf<uint32_t> tot;
+    SJ_TRY(tot.alloc(pool, n));
     SJ_TRY(r->d_offsets.alloc(pool, n + 1));
-    hipLaunchKernelGGL(k_totals_to_orig, dim3(nblk(n)), dim3(JB), 0, s, own.p, mir.p, g->sortedIds.p, n,
-                       cntOrig.p);
+    hipLaunchKernelGGL(k_totals_sorted, dim3(nblk(n)), dim3(JB), 0, s, own.p, mir.p, n, tot.p);
     SJ_LAUNCH_CHECK();
-    SJ_TRY(scan_exclusive_u32_to_u64(pool, s, cntOrig.p, n, r->d_offsets.p));
+    SJ_TRY(scan_exclusive_u32_to_u64(pool, s, tot.p, n, r->d_offsets.p));
     ts.stop();
     uint64_t *pin = ctx->pinned;
     SJ_TRY_HIP(hipMemcpyAsync(pin, r->d_offsets.p + n, 8, hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipMemcpyAsync(pin + 1, stat.p, 3 * 8, hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipMemcpyAsync(pin + 1, stat.p, 8 * 8, hipMemcpyDeviceToHost, s));
     SJ_TRY_HIP(hipStreamSynchronize(s));
     const uint64_t total = pin[0];
     r->total = total;
@@ -863,32 +1146,32 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     r->stats.candidates = pin[1 + STAT_CAND];
     r->stats.fp32_band_pairs = pin[1 + STAT_BAND];
     r->stats.fp32_misclassified = pin[1 + STAT_MIS];
+    r->stats.overflow_retries = 0;
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
+    SJ_TRY(r->d_rowIds.alloc(pool, n));
+    SJ_TRY_HIP(hipMemcpyAsync(r->d_rowIds.p, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
 
-    DBuf<uint64_t> ownBase, rowEnd;
     DBuf<unsigned long long> mirCur;
-    SJ_TRY(ownBase.alloc(pool, n));
-    SJ_TRY(rowEnd.alloc(pool, n));
     SJ_TRY(mirCur.alloc(pool, n));
-    a.ownBase = ownBase.p;
-    a.rowEnd = rowEnd.p;
     a.mirCur = mirCur.p;
+    a.offS = r->d_offsets.p;
 
     const bool whole = P->capacity == 0 || total <= P->capacity;
     double ms_fill = 0.0;
     if (whole) {
         SJ_TRY(r->d_ids.alloc(pool, total));
-        hipLaunchKernelGGL(k_row_bases_orig, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, g->sortedIds.p,
-                           own.p, n, ownBase.p, mirCur.p, rowEnd.p);
+        hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                           (uint32_t)n, 0ull, mirCur.p);
         SJ_LAUNCH_CHECK();
         a.ids = r->d_ids.p;
-        if (a.order) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
+        a.outBase = 0;
+        if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
         tf.start();
-        SJ_TRY(launch_join(g, a, PASS_FILL, prec, (uint32_t)n));
+        SJ_TRY(launch_join(g, a, PASS_FILL, prec));
         tf.stop();
         if (P->sort_rows) {
             tr.start();
-            SJ_TRY(sort_rows_device(ctx, r->d_offsets.p, r->d_ids.p, n));
+            SJ_TRY(sort_rows_device(ctx, r->d_offsets.p, r->d_ids.p, n, 0));
             tr.stop();
         }
         SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + STAT_ERR, 8, hipMemcpyDeviceToHost, s));
@@ -901,27 +1184,18 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         r->on_device = true;
         r->stats.num_batches = 1;
         ms_fill = tf.ms();
-        r->stats.ms_sort_rows = P->sort_rows ? tr.ms() : 0.0;
+        r->stats.ms_sort_rows = tr.ms();
     } else {
-        // Batches of rows in sorted order, each within `capacity` pairs.
-        DBuf<uint32_t> tot;
-        DBuf<uint64_t> offS;
-        SJ_TRY(tot.alloc(pool, n));
-        SJ_TRY(offS.alloc(pool, n + 1));
-        hipLaunchKernelGGL(k_totals_sorted, dim3(nblk(n)), dim3(JB), 0, s, own.p, mir.p, n, tot.p);
-        SJ_LAUNCH_CHECK();
-        SJ_TRY(scan_exclusive_u32_to_u64(pool, s, tot.p, n, offS.p));
+        // Batches: windows of rows in cell order, each within `capacity` pairs.
         std::vector<uint64_t> hOff(n + 1);
         std::vector<uint64_t> hLin(g->nNonEmpty);
-        std::vector<uint32_t> hCellOf(n), hIds(n);
-        SJ_TRY_HIP(hipMemcpyAsync(hOff.data(), offS.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+        std::vector<uint32_t> hCellOf(n), hIds(n), hStart(g->nNonEmpty + 1);
+        SJ_TRY_HIP(hipMemcpyAsync(hOff.data(), r->d_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipMemcpyAsync(hLin.data(), g->cellLinear.p, g->nNonEmpty * 8, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipMemcpyAsync(hCellOf.data(), g->cellOf.p, n * 4, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipMemcpyAsync(hIds.data(), g->sortedIds.p, n * 4, hipMemcpyDeviceToHost, s));
-        std::vector<uint32_t> hStart(g->nNonEmpty + 1);
         SJ_TRY_HIP(hipMemcpyAsync(hStart.data(), g->cellStart.p, (g->nNonEmpty + 1) * 4, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipStreamSynchronize(s));
-        // plan windows
         std::vector<std::pair<uint32_t, uint32_t>> win;
         uint64_t w0 = 0;
         while (w0 < n) {
@@ -943,7 +1217,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         r->batches.resize(win.size());
         for (size_t bi = 0; bi < win.size(); bi++) {
             const uint32_t s0 = win[bi].first, s1 = win[bi].second;
-            // query cells: the window's cells, widened to every cell that may
+            // query range: the window's cells, widened to every cell that may
             // claim one of them (adjacent cells lie within Mmax linear IDs)
             uint64_t q0 = s0, q1 = s1;
             if (P->form != SJ_FORM_FULL) {
@@ -955,31 +1229,31 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
                 q0 = hStart[c0];
                 q1 = hStart[c1];
             }
-            hipLaunchKernelGGL(k_row_bases_window, dim3(nblk(s1 - s0)), dim3(JB), 0, s, offS.p, own.p, s0, s1,
-                               ownBase.p, mirCur.p, rowEnd.p);
+            const uint64_t base = hOff[s0];
+            hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(s1 - s0)), dim3(JB), 0, s, r->d_offsets.p, own.p, s0, s1,
+                               (unsigned long long)base, mirCur.p);
             SJ_LAUNCH_CHECK();
             JoinArgs ab = a;
-            ab.order = nullptr;  // batches use the strided range of their cells
+            ab.tileOrder = nullptr;  // batches walk their query range in order
             ab.qBegin = (uint32_t)q0;
             ab.qEnd = (uint32_t)q1;
+            ab.nTiles = (uint32_t)((q1 - q0 + 63) / 64);
             ab.wBegin = s0;
             ab.wEnd = s1;
+            ab.outBase = base;
             tf.start();
-            SJ_TRY(launch_join(g, ab, PASS_FILL, prec, 0));
+            SJ_TRY(launch_join(g, ab, PASS_FILL, prec));
             tf.stop();
-            const uint64_t cnt = hOff[s1] - hOff[s0];
+            if (P->sort_rows) SJ_TRY(sort_rows_device(ctx, r->d_offsets.p + s0, dbuf.p, s1 - s0, base));
+            const uint64_t cnt = hOff[s1] - base;
             sj_batch_host &B = r->batches[bi];
             B.ids.resize(cnt);
             B.rowIds.assign(hIds.begin() + s0, hIds.begin() + s1);
             B.rowOffsets.resize((size_t)(s1 - s0) + 1);
-            for (uint32_t q = s0; q <= s1; q++) B.rowOffsets[q - s0] = hOff[q] - hOff[s0];
+            for (uint32_t q = s0; q <= s1; q++) B.rowOffsets[q - s0] = hOff[q] - base;
             SJ_TRY_HIP(hipMemcpyAsync(B.ids.data(), dbuf.p, cnt * 4, hipMemcpyDeviceToHost, s));
             SJ_TRY_HIP(hipStreamSynchronize(s));
             ms_fill += tf.ms();
-            if (P->sort_rows) {
-                for (uint32_t q = 0; q < s1 - s0; q++)
-                    std::sort(B.ids.begin() + B.rowOffsets[q], B.ids.begin() + B.rowOffsets[q + 1]);
-            }
         }
         SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + STAT_ERR, 8, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipStreamSynchronize(s));
@@ -989,9 +1263,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             return SJ_ERR_STATE;
         }
         r->on_device = false;
+        r->d_ids.reset();
         r->stats.num_batches = win.size();
     }
-    r->stats.ms_workload = P->schedule == SJ_SCHED_WORKQUEUE ? tw.ms() : 0.0;
+    r->stats.ms_workload = tw.ms();
     r->stats.ms_count = tc.ms();
     r->stats.ms_scan = ts.ms();
     r->stats.ms_fill = ms_fill;
@@ -1005,20 +1280,20 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 // bitonic sort in LDS, longer rows by a serial shell sort (rare).
 constexpr int SR_CAP = 1024;
 
-__global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int32_t *ids, uint64_t n) {
+__global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int32_t *ids, uint64_t n,
+                                                   uint64_t base) {
     __shared__ int32_t buf[4][SR_CAP];
     const unsigned wave = threadIdx.x >> 6, lane = __lane_id();
     const uint64_t row = (uint64_t)blockIdx.x * 4 + wave;
     if (row >= n) return;
-    const uint64_t b = offsets[row], e = offsets[row + 1];
+    const uint64_t b = offsets[row] - base, e = offsets[row + 1] - base;
     const uint64_t len = e - b;
     if (len < 2) return;
     if (len <= SR_CAP) {
         uint32_t p2 = 2;
         while (p2 < len) p2 <<= 1;
         for (uint32_t i = lane; i < p2; i += 64) buf[wave][i] = i < len ? ids[b + i] : INT32_MAX;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        wave_lds_sync();
         for (uint32_t kk = 2; kk <= p2; kk <<= 1) {
             for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
                 for (uint32_t i = lane; i < p2; i += 64) {
@@ -1032,8 +1307,7 @@ __global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int3
                         }
                     }
                 }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                wave_lds_sync();
             }
         }
         for (uint32_t i = lane; i < len; i += 64) ids[b + i] = buf[wave][i];
@@ -1049,10 +1323,48 @@ __global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int3
     }
 }
 
-sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n) {
+sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base) {
     if (!n) return SJ_OK;
-    hipLaunchKernelGGL(k_sort_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, offsets, ids, n);
+    hipLaunchKernelGGL(k_sort_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, offsets, ids, n,
+                       base);
     SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
+// Original-ID-order CSR from the cell-order table (on demand).
+__global__ void k_row_lengths_orig(const uint64_t *off, const uint32_t *rowIds, uint64_t n, uint32_t *len) {
+    uint64_t r = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (r >= n) return;
+    len[rowIds[r]] = (uint32_t)(off[r + 1] - off[r]);
+}
+
+__global__ __launch_bounds__(256) void k_copy_rows_orig(const uint64_t *off, const int32_t *ids,
+                                                        const uint32_t *rowIds, uint64_t n,
+                                                        const uint64_t *offO, int32_t *idsO) {
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const uint64_t b = off[r], e = off[r + 1];
+    const uint64_t d = offO[rowIds[r]];
+    for (uint64_t i = b + __lane_id(); i < e; i += 64) idsO[d + (i - b)] = ids[i];
+}
+
+sj_status result_to_orig(sj_result_s *r) {
+    if (r->d_offsetsOrig.p) return SJ_OK;
+    Pool *pool = &r->ctx->pool;
+    hipStream_t s = r->ctx->stream;
+    const uint64_t n = r->n;
+    DBuf<uint32_t> len;
+    SJ_TRY(len.alloc(pool, n));
+    SJ_TRY(r->d_offsetsOrig.alloc(pool, n + 1));
+    SJ_TRY(r->d_idsOrig.alloc(pool, r->total));
+    hipLaunchKernelGGL(k_row_lengths_orig, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, r->d_rowIds.p, n,
+                       len.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY(scan_exclusive_u32_to_u64(pool, s, len.p, n, r->d_offsetsOrig.p));
+    hipLaunchKernelGGL(k_copy_rows_orig, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, r->d_offsets.p,
+                       r->d_ids.p, r->d_rowIds.p, n, r->d_offsetsOrig.p, r->d_idsOrig.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY_HIP(hipStreamSynchronize(s));
     return SJ_OK;
 }
 

@@ -89,7 +89,7 @@ EXPORTS = [
     "sj_last_error", "sj_context_create", "sj_context_destroy", "sj_context_set_stream",
     "sj_grid_build", "sj_grid_get_info", "sj_grid_copy_arrays", "sj_grid_nd_mask", "sj_grid_free",
     "sj_join", "sj_self_join", "sj_result_get_stats", "sj_result_total_pairs",
-    "sj_result_num_points", "sj_result_device_csr", "sj_result_copy_csr", "sj_result_copy_pairs",
+    "sj_result_num_points", "sj_result_device_table", "sj_result_device_csr", "sj_result_copy_csr", "sj_result_copy_pairs",
     "sj_result_neighbor_table", "sj_result_batch_data", "sj_result_free", "sj_workload",
     "sj_batch_estimate", "sj_import_csv", "sj_sort_in_nd_bins", "sj_free_host", "sj_eps_threshold",
 ]
@@ -128,6 +128,7 @@ def lib():
     L.sj_result_num_points.argtypes = [vp]
     L.sj_result_num_points.restype = u64
     L.sj_result_device_csr.argtypes = [vp, P(vp), P(vp)]
+    L.sj_result_device_table.argtypes = [vp, P(vp), P(vp), P(vp)]
     L.sj_result_copy_csr.argtypes = [vp, vp, vp]
     L.sj_result_copy_pairs.argtypes = [vp, vp, vp]
     L.sj_result_neighbor_table.argtypes = [vp, vp]
@@ -158,13 +159,38 @@ def eps_threshold(eps):
     return lib().sj_eps_threshold(float(eps))
 
 
+_hip_rt = None
+
+
+def _hip():
+    global _hip_rt
+    if _hip_rt is None:
+        h = C.CDLL("libamdhip64.so")
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        _hip_rt = h
+    return _hip_rt
+
+
+def download(ptr, count, dtype):
+    """Copy `count` elements of a device array to a new numpy array."""
+    out = np.empty(count, dtype=dtype)
+    if count:
+        rc = _hip().hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(ptr), out.nbytes, 2)
+        if rc != 0:
+            raise SJError(3, f"hipMemcpy device->host failed ({rc})")
+    return out
+
+
 def make_params(form=LID, precision=FP64, schedule=STRIDED, sort_rows=False, capacity=0,
-                queue_chunk=0):
+                queue_chunk=0, per_point=False):
+    """per_point=True disables the LDS tile walk (every lane walks its
+    neighbour cells from memory); results are identical."""
     if isinstance(form, str):
         form = FORMS[form]
     return JoinParams(form=form, precision=precision, schedule=schedule,
                       sort_rows=1 if sort_rows else 0, capacity=capacity, block_size=0,
-                      queue_chunk=queue_chunk, cells_per_tile=0, reserved=0)
+                      queue_chunk=queue_chunk, cells_per_tile=-1 if per_point else 0, reserved=0)
 
 
 class Context:
@@ -310,6 +336,11 @@ class Result:
         o, i = C.c_void_p(), C.c_void_p()
         _check(lib().sj_result_device_csr(self.h, C.byref(o), C.byref(i)))
         return o.value, i.value
+
+    def device_table(self):
+        o, i, r = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().sj_result_device_table(self.h, C.byref(o), C.byref(i), C.byref(r)))
+        return o.value, i.value, r.value
 
     def csr(self):
         off = np.zeros(self.n + 1, dtype=np.uint64)

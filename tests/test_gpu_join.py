@@ -77,11 +77,12 @@ def test_grid_arrays_match_oracle(ctx, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("form", ["full", "unicomp", "bunicomp", "lid"])
-def test_join_fp64_matches_oracle(ctx, name, form):
+@pytest.mark.parametrize("walk", ["tile", "point"])
+def test_join_fp64_matches_oracle(ctx, name, form, walk):
     pts, k, eps, og, ref = _case(name)
-    _say(f"[join fp64] {name} form={form}")
+    _say(f"[join fp64] {name} form={form} walk={walk}")
     g = ctx.grid_build(pts, k, eps)
-    r = g.join(form=form)
+    r = g.join(form=form, per_point=(walk == "point"))
     _assert_same(r.csr(), ref)
     st = r.stats()
     assert st["total_pairs"] == int(ref[0][-1])
@@ -117,6 +118,27 @@ def test_join_workqueue(ctx, name, form):
     _assert_same(r.csr(), ref)
     r.free()
     g.free()
+
+
+def test_device_views(ctx):
+    """cell-order table + row IDs, and the original-order CSR built from it"""
+    from sjamd import download
+
+    pts, k, eps, og, ref = _case("uniform2d_100k")
+    _say("[device views] uniform2d_100k")
+    r = ctx.self_join(pts, k, eps, form="lid")
+    n, tot = pts.shape[0], r.total
+    o, i, rid = r.device_table()
+    off = download(o, n + 1, np.uint64)
+    ids = download(i, tot, np.int32)
+    rows = download(rid, n, np.uint32).astype(np.int64)
+    np.testing.assert_array_equal(np.sort(rows), np.arange(n))
+    for rr in [0, 5, n // 2, n - 1]:
+        p = rows[rr]
+        got = np.sort(ids[int(off[rr]):int(off[rr + 1])])
+        np.testing.assert_array_equal(got, ref[1][int(ref[0][p]):int(ref[0][p + 1])])
+    o2, i2 = r.device_csr()
+    _assert_same((download(o2, n + 1, np.uint64), download(i2, tot, np.int32)), ref)
 
 
 def test_sorted_rows_device(ctx):
