@@ -74,22 +74,49 @@ __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, in
 
 // main.cu:454-501: min/max, then minArr -= eps, maxArr += eps,
 // nCells = ceil((max-min)/eps), totalCells = prod nCells (dim 0 fastest).
-__global__ void k_grid_params(const double *partial, int nb, int k, double eps, GridParams *gp,
-                              const uint32_t *flags) {
+// One block: the block reduces the per-block partials of each dimension.
+__global__ __launch_bounds__(GB) void k_grid_params(const double *partial, int nb, int k, double eps,
+                                                    GridParams *gp, const uint32_t *flags) {
+    __shared__ double smn[GB / 64][SJ_MAXK], smx[GB / 64][SJ_MAXK];
+    __shared__ double fmn[SJ_MAXK], fmx[SJ_MAXK];
+    const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int j = 0; j < k; j++) {
+        double a = INFINITY, b = -INFINITY;
+        for (int i = threadIdx.x; i < nb; i += GB) {
+            a = fmin(a, partial[(uint64_t)i * 2 * k + j]);
+            b = fmax(b, partial[(uint64_t)i * 2 * k + k + j]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a = fmin(a, __shfl_xor(a, off, 64));
+            b = fmax(b, __shfl_xor(b, off, 64));
+        }
+        if (lane == 0) {
+            smn[wave][j] = a;
+            smx[wave][j] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)k) {
+        const int j = threadIdx.x;
+        double a = INFINITY, b = -INFINITY;
+        for (int w = 0; w < GB / 64; w++) {
+            a = fmin(a, smn[w][j]);
+            b = fmax(b, smx[w][j]);
+        }
+        fmn[j] = a;
+        fmx[j] = b;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     uint32_t f = *flags;
     uint64_t tot = 1, mul = 1;
     for (int j = 0; j < k; j++) {
-        double a = INFINITY, b = -INFINITY;
-        for (int i = 0; i < nb; i++) {
-            a = fmin(a, partial[(uint64_t)i * 2 * k + j]);
-            b = fmax(b, partial[(uint64_t)i * 2 * k + k + j]);
-        }
-        a -= eps;
-        b += eps;
+        const double a = fmn[j] - eps;
+        const double b = fmx[j] + eps;
         gp->minArr[j] = a;
         gp->maxArr[j] = b;
-        double c = ceil((b - a) / eps);
+        const double c = ceil((b - a) / eps);
         uint32_t nc = 0;
         if (!(c < 4294967296.0) || !(c >= 1.0))
             f |= GP_OVERFLOW;
@@ -97,9 +124,8 @@ __global__ void k_grid_params(const double *partial, int nb, int k, double eps, 
             nc = (uint32_t)c;
         gp->nCells[j] = nc;
         gp->mult[j] = mul;
-        unsigned __int128 t = (unsigned __int128)tot * nc;
-        if (t >> 64) f |= GP_OVERFLOW;
-        tot = (uint64_t)t;
+        if (__umul64hi(tot, (uint64_t)nc) != 0) f |= GP_OVERFLOW;
+        tot = tot * (uint64_t)nc;
         mul = tot;
     }
     for (int j = k; j < SJ_MAXK; j++) {
@@ -161,14 +187,16 @@ __global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint
     if (i == n - 1) cellStart[c + 1] = (uint32_t)n;
 }
 
+// one thread per (point, coordinate): row s of the output is row ids[s]
 __global__ __launch_bounds__(GB) void k_gather_points(const double *pts, const uint32_t *ids,
                                                       uint64_t n, int d, double *out) {
-    uint64_t e = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    uint64_t total = n * (uint64_t)d;
-    if (e >= total) return;
-    uint64_t s = e / (uint64_t)d;
-    int j = (int)(e - s * (uint64_t)d);
-    out[e] = pts[(uint64_t)ids[s] * d + j];
+    const uint64_t e0 = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    const uint64_t total = n * (uint64_t)d;
+    if (e0 >= total) return;
+    // 32-bit division when the element count allows it
+    const uint64_t row = total <= 0xFFFFFFFFull ? (uint64_t)((uint32_t)e0 / (uint32_t)d) : e0 / (uint64_t)d;
+    const int j = (int)(e0 - row * d);
+    out[e0] = pts[(uint64_t)ids[row] * d + j];
 }
 
 __global__ __launch_bounds__(GB) void k_cell_coords(const double *spts, const uint32_t *cellStart,
@@ -297,7 +325,7 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     SJ_TRY(fill_u32(s, flags.p, 1, 0u));
     hipLaunchKernelGGL(k_minmax, dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);
     SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(64), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
+    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
                        flags.p);
     SJ_LAUNCH_CHECK();
     SJ_TRY_HIP(hipMemcpyAsync(&g->gp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));

@@ -210,7 +210,7 @@ __device__ __forceinline__ bool decide(const JoinArgs &a, const double *p, const
 // ------------------------------------------------------- per-lane fallback
 // One query point, neighbour cells one at a time straight from memory.
 template <int K, int D, int PASS, int PREC>
-__device__ void process_point(const JoinArgs &a, uint32_t s) {
+__device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s) {
     constexpr int DR = D > 0 ? D : 1;
     const int d = D > 0 ? D : a.d;
     const unsigned lane = __lane_id();
@@ -244,7 +244,7 @@ __device__ void process_point(const JoinArgs &a, uint32_t s) {
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
     const bool anyMirror = a.form != SJ_FORM_FULL;
 
-    auto visit = [&](uint32_t b, bool mirror) {
+    auto visit = [&](uint32_t b, bool mirror) __attribute__((always_inline)) {
         const uint32_t t0 = a.cellStart[b], t1 = a.cellStart[b + 1];
         if (PASS == PASS_COUNT) cand += t1 - t0;
         for (uint32_t t = t0; t < t1; ++t) {
@@ -339,7 +339,7 @@ template <int D, int PREC>
 struct TileCfg {
     static constexpr int DD = D > 0 ? D : 1;
     static constexpr int CH = (PREC == 1 || D <= 4) ? 256 : 128;  // staged candidates per chunk
-    static constexpr int MW = CH / 64;                              // mask words per lane
+    static constexpr int MW = CH / 64;                              // mask words per lane (<= 4)
 };
 
 template <int D, int PREC>
@@ -353,7 +353,7 @@ struct TileLds {
 };
 
 template <int K, int D, int PASS, int PREC>
-__device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> &L) {
+__device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> &L) {
     using Cfg = TileCfg<D, PREC>;
     constexpr int CH = Cfg::CH;
     constexpr int MW = Cfg::MW;
@@ -413,6 +413,30 @@ __device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> 
     }
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
     const bool anyMirror = a.form != SJ_FORM_FULL;
+    // lane-private appender of the own row: single stores up to a 16-byte
+    // boundary, then groups of four IDs as one aligned 16-byte store
+    uint64_t wpos = ownPos;
+    int32_t wb0 = 0, wb1 = 0, wb2 = 0;
+    uint32_t wnb = 0;
+    auto push = [&](int32_t v) __attribute__((always_inline)) {
+        if (wnb == 0 && (wpos & 3u)) {
+            if (wpos < ownLim) a.ids[wpos] = v; else err++;
+        } else if (wnb == 0) {
+            wb0 = v;
+            wnb = 1;
+        } else if (wnb == 1) {
+            wb1 = v;
+            wnb = 2;
+        } else if (wnb == 2) {
+            wb2 = v;
+            wnb = 3;
+        } else {
+            if (wpos < ownLim) *reinterpret_cast<int4 *>(a.ids + (wpos - 3)) = make_int4(wb0, wb1, wb2, v);
+            else err++;
+            wnb = 0;
+        }
+        wpos++;
+    };
 
     // neighbour rows: odometer over dims 1..K-1
     int o[K];
@@ -480,46 +504,104 @@ __device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> 
                         L.cnt[j] = 0u;
                     }
                     wave_lds_sync();
-                    uint64_t mask[MW];
-#pragma unroll
-                    for (int w = 0; w < MW; w++) mask[w] = 0ull;
+                    // per-lane bitmask of mirrored hits over the chunk slots
+                    // (four named words: a runtime-indexed array would spill)
+                    uint64_t mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
                     bool anyMir = false;
-                    // walk the claimed sub-ranges inside this chunk
-#pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        const bool use = r == 0 ? useL : (r == 1 ? useC : useR);
-                        const bool mir = r == 0 ? mirL : (r == 1 ? mirC : mirR);
-                        const uint32_t rs = r == 0 ? e0 : (r == 1 ? e1 : e2);
-                        const uint32_t re = r == 0 ? e1 : (r == 1 ? e2 : e3);
-                        if (!use) continue;
-                        const uint32_t lo = max(rs, c0), hi = min(re, c0 + m);
-                        for (uint32_t t = lo; t < hi; ++t) {
+                    auto emit = [&](uint32_t t, uint32_t j, bool mir) __attribute__((always_inline)) {
+                        if (ownW) {
+                            if (PASS == PASS_FILL) push(L.id[j]);
+                            own++;
+                        }
+                        if (mir && t >= a.wBegin && t < a.wEnd) {
+                            atomicAdd(&L.cnt[j], 1u);
+                            if (PASS == PASS_FILL) {
+                                const uint32_t ws = j >> 6;
+                                const uint64_t bit = 1ull << (j & 63);
+                                mk0 |= ws == 0 ? bit : 0ull;
+                                mk1 |= ws == 1 ? bit : 0ull;
+                                mk2 |= ws == 2 ? bit : 0ull;
+                                mk3 |= ws == 3 ? bit : 0ull;
+                                anyMir = true;
+                            }
+                        }
+                    };
+                    // one candidate, FP32 bulk + FP64 band or plain FP64
+                    auto test1 = [&](uint32_t t, uint32_t j) __attribute__((always_inline)) -> bool {
+                        if constexpr (PREC == 1) {
+                            return decide<D, PASS, PREC>(a, p, p32, &L.x32[j][0], t, band, mis);
+                        } else {
+                            return d2_fp64<D>(p, &L.x64[j][0], d) <= a.T;
+                        }
+                    };
+                    // walk [lo, hi) four candidates at a time: all LDS reads
+                    // issue before the first compare.  mirMode 0: no mirror,
+                    // 1: mirror all, 2: mirror all but the origin cell [e1,e2)
+                    auto walk = [&](uint32_t lo, uint32_t hi, int mirMode) __attribute__((always_inline)) {
+                        uint32_t t = lo;
+                        for (; t + 4 <= hi; t += 4) {
                             const uint32_t j = t - c0;
-                            bool hit;
+                            bool h[4];
                             if constexpr (PREC == 1) {
-                                hit = decide<D, PASS, PREC>(a, p, p32, &L.x32[j][0], t, band, mis);
-                            } else {
-                                hit = d2_fp64<D>(p, &L.x64[j][0], d) <= a.T;
-                            }
-                            if (!hit) continue;
-                            if (ownW) {
-                                if (PASS == PASS_FILL) {
-                                    const uint64_t w = ownPos + own;
-                                    if (w < ownLim) a.ids[w] = L.id[j];
-                                    else err++;
-                                }
-                                own++;
-                            }
-                            if (mir && t >= a.wBegin && t < a.wEnd) {
-                                atomicAdd(&L.cnt[j], 1u);
-                                if (PASS == PASS_FILL) {
+                                float q[4][DD];
 #pragma unroll
-                                    for (int w = 0; w < MW; w++)
-                                        if ((int)(j >> 6) == w) mask[w] |= 1ull << (j & 63);
-                                    anyMir = true;
+                                for (int i = 0; i < 4; i++)
+#pragma unroll
+                                    for (int z = 0; z < DD; z++) q[i][z] = L.x32[j + i][z];
+                                float s32[4];
+                                bool amb = false;
+#pragma unroll
+                                for (int i = 0; i < 4; i++) {
+                                    s32[i] = d2_fp32<D>(p32, q[i], d);
+                                    h[i] = s32[i] < a.lo32;
+                                    amb = amb || (s32[i] >= a.lo32 && s32[i] <= a.hi32);
+                                }
+                                if (amb) {
+#pragma unroll
+                                    for (int i = 0; i < 4; i++) {
+                                        if (s32[i] >= a.lo32 && s32[i] <= a.hi32) {
+                                            h[i] = d2_fp64<D>(p, a.pts + (uint64_t)(t + i) * DD, d) <= a.T;
+                                            if (PASS == PASS_COUNT) {
+                                                band++;
+                                                mis += (h[i] != (s32[i] <= a.T32));
+                                            }
+                                        }
+                                    }
+                                }
+                            } else {
+                                double q[4][DD];
+#pragma unroll
+                                for (int i = 0; i < 4; i++)
+#pragma unroll
+                                    for (int z = 0; z < DD; z++) q[i][z] = L.x64[j + i][z];
+#pragma unroll
+                                for (int i = 0; i < 4; i++) h[i] = d2_fp64<D>(p, q[i], d) <= a.T;
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                if (h[i]) {
+                                    const uint32_t ti = t + i;
+                                    const bool mir = mirMode == 1 || (mirMode == 2 && (ti < e1 || ti >= e2));
+                                    emit(ti, j + i, mir);
                                 }
                             }
                         }
+                        for (; t < hi; ++t) {
+                            const uint32_t j = t - c0;
+                            if (test1(t, j)) {
+                                const bool mir = mirMode == 1 || (mirMode == 2 && (t < e1 || t >= e2));
+                                emit(t, j, mir);
+                            }
+                        }
+                    };
+                    if (useL && useC && useR) {
+                        // the three cells are consecutive: one range
+                        const int mm = !anyMirror ? 0 : (originRow ? 2 : 1);
+                        walk(max(e0, c0), min(e3, c0 + m), mm);
+                    } else {
+                        if (useL) walk(max(e0, c0), min(e1, c0 + m), mirL ? 1 : 0);
+                        if (useC) walk(max(e1, c0), min(e2, c0 + m), mirC ? 1 : 0);
+                        if (useR) walk(max(e2, c0), min(e3, c0 + m), mirR ? 1 : 0);
                     }
                     wave_lds_sync();
                     if (anyMirror) {
@@ -539,7 +621,7 @@ __device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> 
                             if (anyMir) {
 #pragma unroll
                                 for (int w = 0; w < MW; w++) {
-                                    uint64_t mm = mask[w];
+                                    uint64_t mm = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
                                     while (mm) {
                                         const uint32_t j = (uint32_t)(w * 64 + __builtin_ctzll(mm));
                                         mm &= mm - 1;
@@ -575,7 +657,13 @@ __device__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> 
         if (band) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)band);
         if (mis) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)mis);
     } else {
-        if (ownW && own != (ownLim - ownPos)) err++;
+        if (ownW) {
+            const uint64_t b = wpos - wnb;
+            if (wnb > 0) { if (b < ownLim) a.ids[b] = wb0; else err++; }
+            if (wnb > 1) { if (b + 1 < ownLim) a.ids[b + 1] = wb1; else err++; }
+            if (wnb > 2) { if (b + 2 < ownLim) a.ids[b + 2] = wb2; else err++; }
+            if (own != (ownLim - ownPos)) err++;
+        }
         if (err) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)err);
     }
 }
@@ -700,20 +788,30 @@ __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_
     mirCur[s] = offS[s] - outBase + own[s];
 }
 
-__global__ void k_to_fp32(const double *pts, uint64_t n, int d, int k, const GridParams *gp, float *out,
-                          unsigned int *rmaxBits) {
-    uint64_t e = (uint64_t)blockIdx.x * JB + threadIdx.x;
+// FP32 copy of the coordinates relative to the grid origin, and the
+// largest |x - origin| (one atomic per block)
+__global__ __launch_bounds__(JB) void k_to_fp32(const double *pts, uint64_t n, int d, int k,
+                                                const GridParams *gp, float *out, unsigned int *rmaxBits) {
+    __shared__ float wmax[WPB];
+    const uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
     float r = 0.0f;
-    if (e < n * (uint64_t)d) {
-        const int j = (int)(e % (uint64_t)d);
-        const double o = j < k ? gp->minArr[j] : 0.0;
-        const double v = pts[e] - o;
-        out[e] = (float)v;
-        r = (float)fabs(v);
+    if (s < n) {
+        for (int j = 0; j < d; j++) {
+            const double o = j < k ? gp->minArr[j] : 0.0;
+            const double v = pts[s * d + j] - o;
+            out[s * d + j] = (float)v;
+            r = fmaxf(r, (float)fabs(v));
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off, 64));
-    if (__lane_id() == 0) atomicMax(rmaxBits, __float_as_uint(r));
+    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = wmax[0];
+        for (int w = 1; w < WPB; w++) m = fmaxf(m, wmax[w]);
+        atomicMax(rmaxBits, __float_as_uint(m));
+    }
 }
 
 __global__ void k_sample_positions(const uint32_t *sortedIds, uint64_t n, uint64_t stride, uint64_t nsamp,
@@ -1078,7 +1176,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         SJ_TRY(rbits.alloc(pool, 1));
         SJ_TRY(fill_u32(s, rbits.p, 1, 0u));
         if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)g->d));
-        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n * (uint64_t)g->d)), dim3(JB), 0, s, g->sortedPts.p, n, g->d,
+        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d,
                            g->k, g->d_gp.p, g->sortedPts32.p, rbits.p);
         SJ_LAUNCH_CHECK();
         uint64_t *pin = ctx->pinned;
