@@ -54,7 +54,8 @@ struct JoinArgs {
     float lo32, hi32, T32, pad1;
     // query range (sorted positions) and tiles of 64 over it
     uint32_t qBegin, qEnd;
-    uint32_t nTiles, pad2;
+    const uint32_t *tileStart;  // tile t = sorted positions [tileStart[t], tileStart[t+1])
+    const uint32_t *nTilesDev;  // number of tiles (device)
     const uint32_t *tileOrder;  // work queue order of tiles, or null
     uint32_t *qctr;
     // rows written by this launch (sorted positions)
@@ -71,6 +72,66 @@ struct JoinArgs {
 
 __device__ __forceinline__ uint32_t popc_below64(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Per-lane statistics, reduced across the wave once per kernel: per-lane
+// atomics on one address would serialise in L2.
+struct LaneStats {
+    uint64_t cand = 0, band = 0, mis = 0, err = 0, fallback = 0;
+};
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Position of the r-th (0-based) set bit of m; m has more than r set bits.
+__device__ __forceinline__ uint32_t select_bit(uint64_t m, uint32_t r) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
+        if (r >= c) {
+            r -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Bits i in [0,64) with start + i in [A, B).
+__device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint32_t B) {
+    const int64_t lo = max((int64_t)A - (int64_t)start, (int64_t)0);
+    const int64_t hi = min((int64_t)B - (int64_t)start, (int64_t)64);
+    if (hi <= lo) return 0ull;
+    const uint64_t w = (uint64_t)(hi - lo);
+    return (w >= 64 ? ~0ull : ((1ull << w) - 1ull)) << lo;
+}
+
+// Last lane whose exclusive prefix pre is <= h (pre non-decreasing).
+__device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t h) {
+    int ow = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int tst = ow + step;
+        const uint32_t pt = __shfl(pre, tst & 63, 64);
+        if (tst < 64 && pt <= h) ow = tst;
+    }
+    return ow;
+}
+
+__device__ __forceinline__ void flush_stats(const JoinArgs &a, const LaneStats &st) {
+    const uint64_t c = wave_sum_u64(st.cand), b = wave_sum_u64(st.band), m = wave_sum_u64(st.mis);
+    const uint64_t e = wave_sum_u64(st.err), f = wave_sum_u64(st.fallback);
+    if (__lane_id() == 0) {
+        if (c) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)c);
+        if (b) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)b);
+        if (m) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)m);
+        if (e) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)e);
+        if (f) atomicAdd(&a.stat[STAT_FALLBACK], (unsigned long long)f);
+    }
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -210,7 +271,7 @@ __device__ __forceinline__ bool decide(const JoinArgs &a, const double *p, const
 // ------------------------------------------------------- per-lane fallback
 // One query point, neighbour cells one at a time straight from memory.
 template <int K, int D, int PASS, int PREC>
-__device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s) {
+__device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s, LaneStats &st) {
     constexpr int DR = D > 0 ? D : 1;
     const int d = D > 0 ? D : a.d;
     const unsigned lane = __lane_id();
@@ -325,13 +386,13 @@ __device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s) {
     }
     if (PASS == PASS_COUNT) {
         if (ownW) a.ownCnt[s] = own;
-        if (cand) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)cand);
-        if (band) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)band);
-        if (mis) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)mis);
     } else {
         if (ownW && own != (ownLim - ownPos)) err++;
-        if (err) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)err);
     }
+    st.cand += cand;
+    st.band += band;
+    st.mis += mis;
+    st.err += err;
 }
 
 // ------------------------------------------------------------ tile path
@@ -353,14 +414,15 @@ struct TileLds {
 };
 
 template <int K, int D, int PASS, int PREC>
-__device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> &L) {
+__device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, TileLds<D, PREC> &L,
+                                             LaneStats &st) {
     using Cfg = TileCfg<D, PREC>;
     constexpr int CH = Cfg::CH;
-    constexpr int MW = Cfg::MW;
     constexpr int DD = Cfg::DD;
     const unsigned lane = __lane_id();
-    const uint64_t s64 = (uint64_t)a.qBegin + (uint64_t)tile * 64 + lane;
-    const bool valid = s64 < a.qEnd;
+    const uint32_t ts0 = a.tileStart[tile], ts1 = a.tileStart[tile + 1];
+    const uint64_t s64 = (uint64_t)ts0 + lane;
+    const bool valid = s64 < ts1;
     const uint32_t s = valid ? (uint32_t)s64 : 0u;
     const uint64_t vmask = __ballot(valid);
     if (!vmask) return;
@@ -369,7 +431,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
 
     // lanes past the end take the last point's cell so that row-level
     // control flow stays uniform across the wave; they never emit
-    const uint32_t sEff = valid ? s : (uint32_t)(a.qBegin + (uint64_t)tile * 64 + lastL);
+    const uint32_t sEff = valid ? s : ts0 + (uint32_t)lastL;
     uint32_t c[K];
     const uint32_t cellA = a.cellOf[sEff];
 #pragma unroll
@@ -377,11 +439,10 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     const uint64_t linA = a.cellLinear[cellA];
     const uint64_t rowBase = linA - c[0];
     const uint64_t rb0 = __shfl(rowBase, firstL, 64);
-    const uint64_t same = __ballot(valid && rowBase == rb0);
-    if (same != vmask) {
+    if (__ballot(valid && rowBase == rb0) != vmask) {
         // the tile crosses a cell row: per-lane walk
-        if (valid) process_point<K, D, PASS, PREC>(a, s);
-        if (PASS == PASS_COUNT && lane == (unsigned)firstL) atomicAdd(&a.stat[STAT_FALLBACK], 1ull);
+        if (valid) process_point<K, D, PASS, PREC>(a, s, st);
+        if (PASS == PASS_COUNT && lane == (unsigned)firstL) st.fallback++;
         return;
     }
     const uint32_t xa = __shfl(c[0], firstL, 64);
@@ -392,15 +453,17 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     // query point in registers
     double p[DD];
     float p32[DD];
-    const int d = D > 0 ? D : a.d;
-    if constexpr (D > 0) {
-        if (valid) {
 #pragma unroll
-            for (int j = 0; j < D; j++) p[j] = a.pts[(uint64_t)s * D + j];
-            if constexpr (PREC == 1) {
+    for (int j = 0; j < DD; j++) {
+        p[j] = 0.0;
+        p32[j] = 0.0f;
+    }
+    if (valid) {
 #pragma unroll
-                for (int j = 0; j < D; j++) p32[j] = a.pts32[(uint64_t)s * D + j];
-            }
+        for (int j = 0; j < DD; j++) p[j] = a.pts[(uint64_t)s * DD + j];
+        if constexpr (PREC == 1) {
+#pragma unroll
+            for (int j = 0; j < DD; j++) p32[j] = a.pts32[(uint64_t)s * DD + j];
         }
     }
     const bool ownW = valid && s >= a.wBegin && s < a.wEnd;
@@ -413,30 +476,6 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     }
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
     const bool anyMirror = a.form != SJ_FORM_FULL;
-    // lane-private appender of the own row: single stores up to a 16-byte
-    // boundary, then groups of four IDs as one aligned 16-byte store
-    uint64_t wpos = ownPos;
-    int32_t wb0 = 0, wb1 = 0, wb2 = 0;
-    uint32_t wnb = 0;
-    auto push = [&](int32_t v) __attribute__((always_inline)) {
-        if (wnb == 0 && (wpos & 3u)) {
-            if (wpos < ownLim) a.ids[wpos] = v; else err++;
-        } else if (wnb == 0) {
-            wb0 = v;
-            wnb = 1;
-        } else if (wnb == 1) {
-            wb1 = v;
-            wnb = 2;
-        } else if (wnb == 2) {
-            wb2 = v;
-            wnb = 3;
-        } else {
-            if (wpos < ownLim) *reinterpret_cast<int4 *>(a.ids + (wpos - 3)) = make_int4(wb0, wb1, wb2, v);
-            else err++;
-            wnb = 0;
-        }
-        wpos++;
-    };
 
     // neighbour rows: odometer over dims 1..K-1
     int o[K];
@@ -446,7 +485,6 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     int nrow = 1;
 #pragma unroll
     for (int j = 1; j < K; j++) nrow *= 3;
-    if constexpr (K == 1) o[0] = 0;
     for (int it = 0; it < nrow; ++it) {
         bool inb = true, originRow = true;
         int64_t dl = 0;
@@ -485,9 +523,14 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                     e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
                     e3 = x + 1 < nC0 ? pos_lb(a, rb + x + 2) : e2;
                 }
-                const bool mirL = anyMirror, mirR = anyMirror, mirC = anyMirror && !originRow;
+                // this lane's span: from the first to the last claimed cell;
+                // unclaimed cells inside it are masked per candidate
+                uint32_t lo = useL ? e0 : (useC ? e1 : e2);
+                uint32_t hi = useR ? e3 : (useC ? e2 : e1);
+                if (!(useL || useC || useR)) lo = hi = e0;
                 if (PASS == PASS_COUNT && valid)
                     cand += (useL ? e1 - e0 : 0) + (useC ? e2 - e1 : 0) + (useR ? e3 - e2 : 0);
+                const bool mirLR = anyMirror, mirC = anyMirror && !originRow;
                 for (uint32_t c0 = u0; c0 < u1; c0 += CH) {
                     const uint32_t m = min((uint32_t)CH, u1 - c0);
                     // stage the chunk
@@ -504,66 +547,58 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                         L.cnt[j] = 0u;
                     }
                     wave_lds_sync();
-                    // per-lane bitmask of mirrored hits over the chunk slots
-                    // (four named words: a runtime-indexed array would spill)
+                    const uint32_t clo = max(lo, c0);
+                    const uint32_t chi = min(hi, c0 + m);
+                    const uint32_t len = chi > clo ? chi - clo : 0u;
+                    uint32_t maxLen = len;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) maxLen = max(maxLen, (uint32_t)__shfl_xor(maxLen, off, 64));
+                    // mirrored hits of the chunk, one word per 64-candidate piece
                     uint64_t mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
                     bool anyMir = false;
-                    auto emit = [&](uint32_t t, uint32_t j, bool mir) __attribute__((always_inline)) {
-                        if (ownW) {
-                            if (PASS == PASS_FILL) push(L.id[j]);
-                            own++;
-                        }
-                        if (mir && t >= a.wBegin && t < a.wEnd) {
-                            atomicAdd(&L.cnt[j], 1u);
-                            if (PASS == PASS_FILL) {
-                                const uint32_t ws = j >> 6;
-                                const uint64_t bit = 1ull << (j & 63);
-                                mk0 |= ws == 0 ? bit : 0ull;
-                                mk1 |= ws == 1 ? bit : 0ull;
-                                mk2 |= ws == 2 ? bit : 0ull;
-                                mk3 |= ws == 3 ? bit : 0ull;
-                                anyMir = true;
+                    uint32_t piece = 0;
+                    for (uint32_t base = 0; base < maxLen; base += 64, piece++) {
+                        const uint32_t n = min(64u, maxLen - base);
+                        uint64_t mask = 0ull;
+                        // uniform trip count; lanes mask candidates outside
+                        // their claimed cells
+                        for (uint32_t i = 0; i < n; i += 4) {
+                            bool ok[4];
+                            uint32_t jj[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const uint32_t off = base + i + u;
+                                const uint32_t t = clo + off;
+                                const bool in = off < len;
+                                const bool claimed = t < e1 ? useL : (t < e2 ? useC : useR);
+                                ok[u] = in && claimed;
+                                jj[u] = min(t - c0, (uint32_t)(CH - 1));
+                                if (!in) jj[u] = 0;
                             }
-                        }
-                    };
-                    // one candidate, FP32 bulk + FP64 band or plain FP64
-                    auto test1 = [&](uint32_t t, uint32_t j) __attribute__((always_inline)) -> bool {
-                        if constexpr (PREC == 1) {
-                            return decide<D, PASS, PREC>(a, p, p32, &L.x32[j][0], t, band, mis);
-                        } else {
-                            return d2_fp64<D>(p, &L.x64[j][0], d) <= a.T;
-                        }
-                    };
-                    // walk [lo, hi) four candidates at a time: all LDS reads
-                    // issue before the first compare.  mirMode 0: no mirror,
-                    // 1: mirror all, 2: mirror all but the origin cell [e1,e2)
-                    auto walk = [&](uint32_t lo, uint32_t hi, int mirMode) __attribute__((always_inline)) {
-                        uint32_t t = lo;
-                        for (; t + 4 <= hi; t += 4) {
-                            const uint32_t j = t - c0;
                             bool h[4];
                             if constexpr (PREC == 1) {
                                 float q[4][DD];
 #pragma unroll
-                                for (int i = 0; i < 4; i++)
+                                for (int u = 0; u < 4; u++)
 #pragma unroll
-                                    for (int z = 0; z < DD; z++) q[i][z] = L.x32[j + i][z];
+                                    for (int z = 0; z < DD; z++) q[u][z] = L.x32[jj[u]][z];
                                 float s32[4];
                                 bool amb = false;
 #pragma unroll
-                                for (int i = 0; i < 4; i++) {
-                                    s32[i] = d2_fp32<D>(p32, q[i], d);
-                                    h[i] = s32[i] < a.lo32;
-                                    amb = amb || (s32[i] >= a.lo32 && s32[i] <= a.hi32);
+                                for (int u = 0; u < 4; u++) {
+                                    s32[u] = d2_fp32<D>(p32, q[u], DD);
+                                    h[u] = ok[u] && s32[u] < a.lo32;
+                                    amb = amb || (ok[u] && s32[u] >= a.lo32 && s32[u] <= a.hi32);
                                 }
                                 if (amb) {
 #pragma unroll
-                                    for (int i = 0; i < 4; i++) {
-                                        if (s32[i] >= a.lo32 && s32[i] <= a.hi32) {
-                                            h[i] = d2_fp64<D>(p, a.pts + (uint64_t)(t + i) * DD, d) <= a.T;
+                                    for (int u = 0; u < 4; u++) {
+                                        if (ok[u] && s32[u] >= a.lo32 && s32[u] <= a.hi32) {
+                                            const uint64_t t = (uint64_t)clo + base + i + u;
+                                            h[u] = d2_fp64<D>(p, a.pts + t * DD, DD) <= a.T;
                                             if (PASS == PASS_COUNT) {
                                                 band++;
-                                                mis += (h[i] != (s32[i] <= a.T32));
+                                                mis += (h[u] != (s32[u] <= a.T32));
                                             }
                                         }
                                     }
@@ -571,37 +606,67 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                             } else {
                                 double q[4][DD];
 #pragma unroll
-                                for (int i = 0; i < 4; i++)
+                                for (int u = 0; u < 4; u++)
 #pragma unroll
-                                    for (int z = 0; z < DD; z++) q[i][z] = L.x64[j + i][z];
+                                    for (int z = 0; z < DD; z++) q[u][z] = L.x64[jj[u]][z];
 #pragma unroll
-                                for (int i = 0; i < 4; i++) h[i] = d2_fp64<D>(p, q[i], d) <= a.T;
+                                for (int u = 0; u < 4; u++) h[u] = ok[u] && d2_fp64<D>(p, q[u], DD) <= a.T;
                             }
 #pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                if (h[i]) {
-                                    const uint32_t ti = t + i;
-                                    const bool mir = mirMode == 1 || (mirMode == 2 && (ti < e1 || ti >= e2));
-                                    emit(ti, j + i, mir);
+                            for (int u = 0; u < 4; u++) mask |= (uint64_t)h[u] << (i + u);
+                        }
+                        // emission, wave-cooperative: the hits of all lanes
+                        // are taken 64 at a time in (lane, candidate) order,
+                        // so consecutive lanes fill consecutive slots of a row
+                        const uint32_t pstart = clo + base;
+                        uint64_t mirBits = 0ull;
+                        if (anyMirror) {
+                            uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
+                            if (mirC) elig |= range_bits(pstart, e1, e2);
+                            mirBits = mask & elig & range_bits(pstart, a.wBegin, a.wEnd);
+                        }
+                        const uint32_t myCnt = (uint32_t)__popcll(mask);
+                        const bool anyMirBits = __ballot(mirBits != 0ull) != 0ull;
+                        if (PASS == PASS_FILL || anyMirBits) {
+                            uint32_t incl = myCnt;
+#pragma unroll
+                            for (int off = 1; off < 64; off <<= 1) {
+                                const uint32_t u = __shfl_up(incl, off, 64);
+                                if (lane >= (unsigned)off) incl += u;
+                            }
+                            const uint32_t pre = incl - myCnt;
+                            const uint32_t tot = __shfl(incl, 63, 64);
+                            const uint64_t wbase = ownPos + own;  // next own slot before this piece
+                            for (uint32_t it = 0; it < tot; it += 64) {
+                                const uint32_t h = it + lane;
+                                const int ow = owner_lane(pre, h);
+                                const uint32_t rank = h - __shfl(pre, ow, 64);
+                                const uint64_t om = __shfl(mask, ow, 64);
+                                const uint64_t omir = __shfl(mirBits, ow, 64);
+                                const uint32_t ostart = __shfl(pstart, ow, 64);
+                                const uint64_t obase = __shfl(wbase, ow, 64);
+                                const uint64_t olim = __shfl(ownLim, ow, 64);
+                                const bool oown = __shfl((int)ownW, ow, 64) != 0;
+                                if (h < tot) {
+                                    const uint32_t b = select_bit(om, rank);
+                                    const uint32_t j = ostart + b - c0;
+                                    if (PASS == PASS_FILL && oown) {
+                                        const uint64_t w = obase + rank;
+                                        if (w < olim) a.ids[w] = L.id[j];
+                                        else err++;
+                                    }
+                                    if ((omir >> b) & 1ull) atomicAdd(&L.cnt[j], 1u);
                                 }
                             }
                         }
-                        for (; t < hi; ++t) {
-                            const uint32_t j = t - c0;
-                            if (test1(t, j)) {
-                                const bool mir = mirMode == 1 || (mirMode == 2 && (t < e1 || t >= e2));
-                                emit(t, j, mir);
-                            }
+                        if (ownW) own += myCnt;
+                        if (PASS == PASS_FILL && mirBits) {
+                            anyMir = true;
+                            if (piece == 0) mk0 = mirBits;
+                            else if (piece == 1) mk1 = mirBits;
+                            else if (piece == 2) mk2 = mirBits;
+                            else mk3 = mirBits;
                         }
-                    };
-                    if (useL && useC && useR) {
-                        // the three cells are consecutive: one range
-                        const int mm = !anyMirror ? 0 : (originRow ? 2 : 1);
-                        walk(max(e0, c0), min(e3, c0 + m), mm);
-                    } else {
-                        if (useL) walk(max(e0, c0), min(e1, c0 + m), mirL ? 1 : 0);
-                        if (useC) walk(max(e1, c0), min(e2, c0 + m), mirC ? 1 : 0);
-                        if (useR) walk(max(e2, c0), min(e3, c0 + m), mirR ? 1 : 0);
                     }
                     wave_lds_sync();
                     if (anyMirror) {
@@ -618,17 +683,35 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                                 L.cnt[j] = 0u;
                             }
                             wave_lds_sync();
-                            if (anyMir) {
+                            // place the mirrored hits, piece by piece, cooperatively
+                            if (__ballot(anyMir)) {
+                                for (int w = 0; w < 4; w++) {
+                                    const uint64_t mb = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
+                                    const uint32_t myCnt = (uint32_t)__popcll(mb);
+                                    uint32_t incl = myCnt;
 #pragma unroll
-                                for (int w = 0; w < MW; w++) {
-                                    uint64_t mm = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
-                                    while (mm) {
-                                        const uint32_t j = (uint32_t)(w * 64 + __builtin_ctzll(mm));
-                                        mm &= mm - 1;
-                                        const uint32_t k2 = atomicAdd(&L.cnt[j], 1u);
-                                        const uint64_t wpos = L.base[j] + k2;
-                                        if (wpos < a.offS[c0 + j + 1] - a.outBase) a.ids[wpos] = myId;
-                                        else err++;
+                                    for (int off = 1; off < 64; off <<= 1) {
+                                        const uint32_t u = __shfl_up(incl, off, 64);
+                                        if (lane >= (unsigned)off) incl += u;
+                                    }
+                                    const uint32_t pre = incl - myCnt;
+                                    const uint32_t tot = __shfl(incl, 63, 64);
+                                    const uint32_t pst = clo + (uint32_t)w * 64;
+                                    for (uint32_t it = 0; it < tot; it += 64) {
+                                        const uint32_t h = it + lane;
+                                        const int ow = owner_lane(pre, h);
+                                        const uint32_t rank = h - __shfl(pre, ow, 64);
+                                        const uint64_t om = __shfl(mb, ow, 64);
+                                        const uint32_t ostart = __shfl(pst, ow, 64);
+                                        const int32_t oid = __shfl(myId, ow, 64);
+                                        if (h < tot) {
+                                            const uint32_t b = select_bit(om, rank);
+                                            const uint32_t j = ostart + b - c0;
+                                            const uint32_t k2 = atomicAdd(&L.cnt[j], 1u);
+                                            const uint64_t wp = L.base[j] + k2;
+                                            if (wp < a.offS[c0 + j + 1] - a.outBase) a.ids[wp] = oid;
+                                            else err++;
+                                        }
                                     }
                                 }
                             }
@@ -653,50 +736,68 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     }
     if (PASS == PASS_COUNT) {
         if (ownW) a.ownCnt[s] = own;
-        if (cand) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)cand);
-        if (band) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)band);
-        if (mis) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)mis);
     } else {
-        if (ownW) {
-            const uint64_t b = wpos - wnb;
-            if (wnb > 0) { if (b < ownLim) a.ids[b] = wb0; else err++; }
-            if (wnb > 1) { if (b + 1 < ownLim) a.ids[b + 1] = wb1; else err++; }
-            if (wnb > 2) { if (b + 2 < ownLim) a.ids[b + 2] = wb2; else err++; }
-            if (own != (ownLim - ownPos)) err++;
-        }
-        if (err) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)err);
+        if (ownW && own != (ownLim - ownPos)) err++;
     }
+    st.cand += cand;
+    st.band += band;
+    st.mis += mis;
+    st.err += err;
 }
 
 template <int K, int D, int PASS, int PREC>
 __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
     __shared__ TileLds<D, PREC> lds[WPB];
     const unsigned wave = threadIdx.x >> 6;
+    const unsigned lane = __lane_id();
     TileLds<D, PREC> &L = lds[wave];
+    const uint32_t nTiles = *a.nTilesDev;
+    LaneStats st;
+    auto run = [&](uint32_t tile) __attribute__((always_inline)) {
+        if (D > 0 && a.useTiles) {
+            process_tile<K, D, PASS, PREC>(a, tile, L, st);
+        } else {
+            const uint64_t s = (uint64_t)a.tileStart[tile] + lane;
+            if (s < a.tileStart[tile + 1]) process_point<K, D, PASS, PREC>(a, (uint32_t)s, st);
+        }
+    };
     if (a.tileOrder) {
-        const unsigned lane = __lane_id();
+        // work queue: tiles in descending cost, one per wave per grab
         for (;;) {
             uint32_t idx = 0;
             if (lane == 0) idx = atomicAdd(a.qctr, 1u);
             idx = __shfl(idx, 0, 64);
-            if (idx >= a.nTiles) break;
-            const uint32_t tile = a.tileOrder[idx];
-            if (D > 0 && a.useTiles) {
-                process_tile<K, D, PASS, PREC>(a, tile, L);
-            } else {
-                const uint64_t s = (uint64_t)a.qBegin + (uint64_t)tile * 64 + lane;
-                if (s < a.qEnd) process_point<K, D, PASS, PREC>(a, (uint32_t)s);
-            }
+            if (idx >= nTiles) break;
+            run(a.tileOrder[idx]);
         }
     } else {
-        const uint32_t tile = blockIdx.x * WPB + wave;
-        if (tile >= a.nTiles) return;
-        if (D > 0 && a.useTiles) {
-            process_tile<K, D, PASS, PREC>(a, tile, L);
-        } else {
-            const uint64_t s = (uint64_t)a.qBegin + (uint64_t)tile * 64 + __lane_id();
-            if (s < a.qEnd) process_point<K, D, PASS, PREC>(a, (uint32_t)s);
-        }
+        // persistent grid-stride over the tiles in cell order
+        for (uint32_t tile = blockIdx.x * WPB + wave; tile < nTiles; tile += gridDim.x * WPB) run(tile);
+    }
+    flush_stats(a, st);
+}
+
+// Tiles: runs of at most 64 consecutive points of one cell row within
+// [q0, q1).  A head starts every row and every 64 points inside a row.
+__global__ __launch_bounds__(JB) void k_tile_heads(JoinArgs a, uint32_t q0, uint32_t q1, uint32_t *head) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    const uint64_t s = (uint64_t)q0 + i;
+    if (s >= q1) return;
+    const uint32_t cell = a.cellOf[s];
+    const uint64_t row = a.cellLinear[cell] - a.cellCoord[(uint64_t)cell * a.k];
+    const uint32_t rs = max(pos_lb(a, row), q0);
+    head[i] = ((s - rs) & 63u) == 0 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(JB) void k_tile_compact(const uint32_t *head, const uint32_t *ex, uint32_t q0,
+                                                     uint32_t q1, uint32_t *tileStart, uint32_t *nTiles) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    const uint64_t len = q1 - q0;
+    if (i >= len) return;
+    if (head[i]) tileStart[ex[i]] = q0 + (uint32_t)i;
+    if (i == len - 1) {
+        tileStart[ex[len]] = q1;
+        *nTiles = ex[len];
     }
 }
 
@@ -759,15 +860,18 @@ __global__ void k_desc_keys(const uint32_t *w, uint64_t n, uint32_t *key, uint32
     val[i] = (uint32_t)i;
 }
 
-// tile cost: candidates of its points (cell workload per point)
-__global__ void k_tile_cost(const uint32_t *w, const uint32_t *cellOf, uint32_t q0, uint32_t q1,
-                            uint32_t nTiles, uint32_t *key, uint32_t *val) {
-    const uint64_t s = (uint64_t)q0 + (uint64_t)blockIdx.x * JB + threadIdx.x;
-    const uint32_t tile = (blockIdx.x * JB + threadIdx.x) >> 6;
-    uint32_t v = s < q1 ? w[cellOf[s]] : 0u;
+// tile cost: candidates of its points (cell workload per point); one wave
+// per tile
+__global__ __launch_bounds__(JB) void k_tile_cost(const uint32_t *w, const uint32_t *cellOf,
+                                                  const uint32_t *tileStart, uint32_t nTiles, uint32_t *key,
+                                                  uint32_t *val) {
+    const uint32_t tile = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (tile >= nTiles) return;
+    const uint64_t s = (uint64_t)tileStart[tile] + __lane_id();
+    uint32_t v = s < tileStart[tile + 1] ? w[cellOf[s]] : 0u;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (__lane_id() == 0 && tile < nTiles) {
+    if (__lane_id() == 0) {
         key[tile] = 0xFFFFFFFFu - v;
         val[tile] = tile;
     }
@@ -828,8 +932,9 @@ __global__ void k_sample_positions(const uint32_t *sortedIds, uint64_t n, uint64
 template <int K, int D>
 __global__ __launch_bounds__(JB) void k_sample_count(JoinArgs a, const uint32_t *list, uint64_t m) {
     const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (i >= m) return;
-    process_point<K, D, PASS_COUNT, 0>(a, list[i]);
+    LaneStats st;
+    if (i < m) process_point<K, D, PASS_COUNT, 0>(a, list[i], st);
+    flush_stats(a, st);
 }
 
 __global__ void k_sum_gather(const uint32_t *v, const uint32_t *idx, uint64_t m, unsigned long long *out) {
@@ -1014,7 +1119,6 @@ static JoinArgs base_args(sj_grid_s *g, int form, double T) {
     a.T = T;
     a.qBegin = 0;
     a.qEnd = (uint32_t)g->n;
-    a.nTiles = (uint32_t)((g->n + 63) / 64);
     a.wBegin = 0;
     a.wEnd = (uint32_t)g->n;
     return a;
@@ -1031,18 +1135,48 @@ static sj_status cell_workload(sj_grid_s *g, int lid, DBuf<uint32_t> &w) {
     return SJ_OK;
 }
 
-// Work queue: 64-point tiles in descending candidate count.
-static sj_status build_tile_queue(sj_grid_s *g, int lid, uint32_t nTiles, DBuf<uint32_t> &order) {
+// Row-aligned tiles over [q0, q1); the count stays on the device.
+static sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
+                             DBuf<uint32_t> &nTiles) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
+    const uint64_t len = q1 - q0;
+    DBuf<uint32_t> head, ex;
+    SJ_TRY(head.alloc(pool, len));
+    SJ_TRY(ex.alloc(pool, len + 1));
+    SJ_TRY(tileStart.alloc(pool, len + 1));
+    SJ_TRY(nTiles.alloc(pool, 1));
+    if (len == 0) {
+        SJ_TRY(fill_u32(s, nTiles.p, 1, 0u));
+        return SJ_OK;
+    }
+    hipLaunchKernelGGL(k_tile_heads, dim3(nblk(len)), dim3(JB), 0, s, a, q0, q1, head.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY(scan_exclusive_u32_to_u32(pool, s, head.p, len, ex.p));
+    hipLaunchKernelGGL(k_tile_compact, dim3(nblk(len)), dim3(JB), 0, s, head.p, ex.p, q0, q1, tileStart.p,
+                       nTiles.p);
+    SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
+// Work queue: tiles in descending candidate count.
+static sj_status build_tile_queue(sj_grid_s *g, int lid, const DBuf<uint32_t> &tileStart,
+                                  const DBuf<uint32_t> &nTilesDev, DBuf<uint32_t> &order) {
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    uint64_t *pin = g->ctx->pinned;
+    SJ_TRY_HIP(hipMemcpyAsync(pin, nTilesDev.p, 4, hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipStreamSynchronize(s));
+    const uint32_t nTiles = (uint32_t)(pin[0] & 0xffffffffu);
     DBuf<uint32_t> w, key, key2, val2;
     SJ_TRY(cell_workload(g, lid, w));
     SJ_TRY(key.alloc(pool, nTiles));
     SJ_TRY(key2.alloc(pool, nTiles));
     SJ_TRY(order.alloc(pool, nTiles));
     SJ_TRY(val2.alloc(pool, nTiles));
-    hipLaunchKernelGGL(k_tile_cost, dim3(nblk((uint64_t)nTiles * 64)), dim3(JB), 0, s, w.p, g->cellOf.p, 0u,
-                       (uint32_t)g->n, nTiles, key.p, order.p);
+    if (nTiles == 0) return SJ_OK;
+    hipLaunchKernelGGL(k_tile_cost, dim3((nTiles + WPB - 1) / WPB), dim3(JB), 0, s, w.p, g->cellOf.p,
+                       tileStart.p, nTiles, key.p, order.p);
     SJ_LAUNCH_CHECK();
     uint32_t *kp = key.p, *ka = key2.p, *vp = order.p, *va = val2.p;
     SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nTiles, 32));
@@ -1084,13 +1218,12 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
         set_error("no traversal kernel for k=%d", g->k);
         return SJ_ERR_UNSUPPORTED;
     }
-    unsigned blocks;
-    if (a.tileOrder)
-        blocks = (unsigned)std::min<uint64_t>((uint64_t)g->ctx->cu_count * 8, (a.nTiles + WPB - 1) / WPB);
-    else
-        blocks = (a.nTiles + WPB - 1) / WPB;
+    // persistent waves: enough to fill every CU, at most one per tile bound
+    const uint64_t len = (uint64_t)a.qEnd - a.qBegin;
+    const uint64_t maxTiles = len;  // one point per tile at worst
+    uint64_t blocks = std::min<uint64_t>((uint64_t)g->ctx->cu_count * 16, (maxTiles + WPB - 1) / WPB);
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(kfn, dim3(blocks), dim3(JB), 0, s, a);
+    hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(JB), 0, s, a);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
@@ -1198,11 +1331,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         }
     }
 
-    // scheduling: work queue of tiles in descending workload
-    DBuf<uint32_t> tileOrder, qctr;
+    // row-aligned tiles; work queue of tiles in descending workload
+    DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr;
+    SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev));
+    a.tileStart = tileStart.p;
+    a.nTilesDev = nTilesDev.p;
     if (P->schedule == SJ_SCHED_WORKQUEUE) {
         tw.start();
-        SJ_TRY(build_tile_queue(g, P->form == SJ_FORM_LID ? 1 : 0, a.nTiles, tileOrder));
+        SJ_TRY(build_tile_queue(g, P->form == SJ_FORM_LID ? 1 : 0, tileStart, nTilesDev, tileOrder));
         tw.stop();
         SJ_TRY(qctr.alloc(pool, 1));
         a.tileOrder = tileOrder.p;
@@ -1335,7 +1471,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             ab.tileOrder = nullptr;  // batches walk their query range in order
             ab.qBegin = (uint32_t)q0;
             ab.qEnd = (uint32_t)q1;
-            ab.nTiles = (uint32_t)((q1 - q0 + 63) / 64);
+            DBuf<uint32_t> bStart, bCount;
+            SJ_TRY(build_tiles(g, ab, (uint32_t)q0, (uint32_t)q1, bStart, bCount));
+            ab.tileStart = bStart.p;
+            ab.nTilesDev = bCount.p;
             ab.wBegin = s0;
             ab.wEnd = s1;
             ab.outBase = base;
