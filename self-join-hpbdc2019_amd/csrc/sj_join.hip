@@ -14,10 +14,12 @@
 //   count: hits each query point owns (own row) and, for the Unicomp
 //          forms, mirrored hits given to the candidate (LDS counters per
 //          staged candidate, one global atomic per candidate and chunk);
-//   fill:  own hits go to a lane-private cursor in the point's row; for
-//          mirrored hits each lane keeps a bitmask over the staged chunk,
-//          the wave reserves one range per candidate with one atomic, and
-//          lanes place their IDs through an LDS cursor.
+//   fill:  own hits leave one source lane at a time: the lane's hit mask is
+//          broadcast and each thread stores one hit, so a lane's hits form
+//          one contiguous row write; for mirrored hits each lane keeps a
+//          bitmask over the staged chunk, the wave reserves one range per
+//          candidate with one atomic, and the hits are placed through an
+//          LDS cursor per candidate.
 // Rows are laid out in cell order, so a tile writes one contiguous block.
 // Exact predicate: acc = sum fma((x_q - x_p)^2) in FP64, dimension order;
 // hit <=> acc <= T with T = max{x : sqrt_rn(x) <= eps}.
@@ -86,21 +88,6 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// Position of the r-th (0-based) set bit of m; m has more than r set bits.
-__device__ __forceinline__ uint32_t select_bit(uint64_t m, uint32_t r) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
-        if (r >= c) {
-            r -= c;
-            m >>= w;
-            pos += w;
-        }
-    }
-    return pos;
-}
-
 // Bits i in [0,64) with start + i in [A, B).
 __device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint32_t B) {
     const int64_t lo = max((int64_t)A - (int64_t)start, (int64_t)0);
@@ -110,17 +97,15 @@ __device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint3
     return (w >= 64 ? ~0ull : ((1ull << w) - 1ull)) << lo;
 }
 
-// Last lane whose exclusive prefix pre is <= h (pre non-decreasing).
-__device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t h) {
-    int ow = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        const int tst = ow + step;
-        const uint32_t pt = __shfl(pre, tst & 63, 64);
-        if (tst < 64 && pt <= h) ow = tst;
-    }
-    return ow;
+// Value of v in lane l (l wave-uniform).
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+    return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
+}
+
+constexpr unsigned long long NO_SLOT = ~0ull;
 
 __device__ __forceinline__ void flush_stats(const JoinArgs &a, const LaneStats &st) {
     const uint64_t c = wave_sum_u64(st.cand), b = wave_sum_u64(st.band), m = wave_sum_u64(st.mis);
@@ -615,9 +600,10 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
 #pragma unroll
                             for (int u = 0; u < 4; u++) mask |= (uint64_t)h[u] << (i + u);
                         }
-                        // emission, wave-cooperative: the hits of all lanes
-                        // are taken 64 at a time in (lane, candidate) order,
-                        // so consecutive lanes fill consecutive slots of a row
+                        // emission, one lane at a time: the lane's hit mask is
+                        // broadcast and each thread takes one candidate bit, so
+                        // the lane's hits leave as one contiguous row write;
+                        // mirrored hits bump the candidate's LDS counter
                         const uint32_t pstart = clo + base;
                         uint64_t mirBits = 0ull;
                         if (anyMirror) {
@@ -626,36 +612,49 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                             mirBits = mask & elig & range_bits(pstart, a.wBegin, a.wEnd);
                         }
                         const uint32_t myCnt = (uint32_t)__popcll(mask);
-                        const bool anyMirBits = __ballot(mirBits != 0ull) != 0ull;
-                        if (PASS == PASS_FILL || anyMirBits) {
-                            uint32_t incl = myCnt;
+                        uint64_t ownBits = 0ull;
+                        if (PASS == PASS_FILL && ownW) {
+                            // more hits than the row has room for: count and
+                            // fill disagree; drop the piece and report it
+                            if (ownPos + own + myCnt <= ownLim) ownBits = mask;
+                            else err++;
+                        }
+                        const uint64_t wcur = ownPos + own;
+                        uint64_t todo = __ballot(ownBits != 0ull || mirBits != 0ull);
+                        // four source lanes per round so that their LDS reads overlap
+                        while (todo) {
+                            int l[4];
+                            bool lv[4];
 #pragma unroll
-                            for (int off = 1; off < 64; off <<= 1) {
-                                const uint32_t u = __shfl_up(incl, off, 64);
-                                if (lane >= (unsigned)off) incl += u;
+                            for (int u = 0; u < 4; u++) {
+                                lv[u] = todo != 0ull;
+                                l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+                                todo &= todo - 1ull;
                             }
-                            const uint32_t pre = incl - myCnt;
-                            const uint32_t tot = __shfl(incl, 63, 64);
-                            const uint64_t wbase = ownPos + own;  // next own slot before this piece
-                            for (uint32_t it = 0; it < tot; it += 64) {
-                                const uint32_t h = it + lane;
-                                const int ow = owner_lane(pre, h);
-                                const uint32_t rank = h - __shfl(pre, ow, 64);
-                                const uint64_t om = __shfl(mask, ow, 64);
-                                const uint64_t omir = __shfl(mirBits, ow, 64);
-                                const uint32_t ostart = __shfl(pstart, ow, 64);
-                                const uint64_t obase = __shfl(wbase, ow, 64);
-                                const uint64_t olim = __shfl(ownLim, ow, 64);
-                                const bool oown = __shfl((int)ownW, ow, 64) != 0;
-                                if (h < tot) {
-                                    const uint32_t b = select_bit(om, rank);
-                                    const uint32_t j = ostart + b - c0;
-                                    if (PASS == PASS_FILL && oown) {
-                                        const uint64_t w = obase + rank;
-                                        if (w < olim) a.ids[w] = L.id[j];
-                                        else err++;
-                                    }
-                                    if ((omir >> b) & 1ull) atomicAdd(&L.cnt[j], 1u);
+                            uint64_t oB[4];
+                            uint32_t j[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                oB[u] = lv[u] ? readlane_u64(ownBits, l[u]) : 0ull;
+                                j[u] = min(readlane_u32(pstart, l[u]) + lane - c0, (uint32_t)(CH - 1));
+                            }
+                            if (PASS == PASS_FILL) {
+                                int32_t idv[4];
+                                uint64_t w[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    idv[u] = L.id[j[u]];
+                                    w[u] = readlane_u64(wcur, l[u]) + popc_below64(oB[u]);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 4; u++)
+                                    if ((oB[u] >> lane) & 1ull) a.ids[w[u]] = idv[u];
+                            }
+                            if (anyMirror) {
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const uint64_t mB = lv[u] ? readlane_u64(mirBits, l[u]) : 0ull;
+                                    if ((mB >> lane) & 1ull) atomicAdd(&L.cnt[j[u]], 1u);
                                 }
                             }
                         }
@@ -676,42 +675,58 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                                 if (v) atomicAdd(&a.mirCnt[c0 + j], v);
                             }
                         } else {
-                            // reserve one range per candidate, then place
+                            // reserve one range per candidate, then place; a
+                            // reservation past the candidate's row end is an
+                            // error and its hits are dropped
                             for (uint32_t j = lane; j < m; j += 64) {
                                 const uint32_t v = L.cnt[j];
-                                if (v) L.base[j] = atomicAdd(&a.mirCur[c0 + j], (unsigned long long)v);
+                                if (v) {
+                                    unsigned long long b = atomicAdd(&a.mirCur[c0 + j], (unsigned long long)v);
+                                    if (b + v > a.offS[c0 + j + 1] - a.outBase) {
+                                        err++;
+                                        b = NO_SLOT;
+                                    }
+                                    L.base[j] = b;
+                                }
                                 L.cnt[j] = 0u;
                             }
                             wave_lds_sync();
-                            // place the mirrored hits, piece by piece, cooperatively
+                            // place the mirrored hits, one source lane at a time
                             if (__ballot(anyMir)) {
-                                for (int w = 0; w < 4; w++) {
-                                    const uint64_t mb = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
-                                    const uint32_t myCnt = (uint32_t)__popcll(mb);
-                                    uint32_t incl = myCnt;
 #pragma unroll
-                                    for (int off = 1; off < 64; off <<= 1) {
-                                        const uint32_t u = __shfl_up(incl, off, 64);
-                                        if (lane >= (unsigned)off) incl += u;
-                                    }
-                                    const uint32_t pre = incl - myCnt;
-                                    const uint32_t tot = __shfl(incl, 63, 64);
-                                    const uint32_t pst = clo + (uint32_t)w * 64;
-                                    for (uint32_t it = 0; it < tot; it += 64) {
-                                        const uint32_t h = it + lane;
-                                        const int ow = owner_lane(pre, h);
-                                        const uint32_t rank = h - __shfl(pre, ow, 64);
-                                        const uint64_t om = __shfl(mb, ow, 64);
-                                        const uint32_t ostart = __shfl(pst, ow, 64);
-                                        const int32_t oid = __shfl(myId, ow, 64);
-                                        if (h < tot) {
-                                            const uint32_t b = select_bit(om, rank);
-                                            const uint32_t j = ostart + b - c0;
-                                            const uint32_t k2 = atomicAdd(&L.cnt[j], 1u);
-                                            const uint64_t wp = L.base[j] + k2;
-                                            if (wp < a.offS[c0 + j + 1] - a.outBase) a.ids[wp] = oid;
-                                            else err++;
+                                for (int w = 0; w < Cfg::MW; w++) {
+                                    const uint64_t mb = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
+                                    uint64_t todo = __ballot(mb != 0ull);
+                                    while (todo) {
+                                        int l[4];
+                                        bool lv[4];
+#pragma unroll
+                                        for (int u = 0; u < 4; u++) {
+                                            lv[u] = todo != 0ull;
+                                            l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+                                            todo &= todo - 1ull;
                                         }
+                                        bool hb[4];
+                                        uint32_t j[4];
+                                        int32_t oid[4];
+#pragma unroll
+                                        for (int u = 0; u < 4; u++) {
+                                            const uint64_t bits = lv[u] ? readlane_u64(mb, l[u]) : 0ull;
+                                            hb[u] = (bits >> lane) & 1ull;
+                                            j[u] = min(readlane_u32(clo, l[u]) + (uint32_t)w * 64u + lane - c0,
+                                                       (uint32_t)(CH - 1));
+                                            oid[u] = (int32_t)readlane_u32((uint32_t)myId, l[u]);
+                                        }
+                                        uint32_t k2[4];
+                                        unsigned long long bb[4];
+#pragma unroll
+                                        for (int u = 0; u < 4; u++) {
+                                            k2[u] = hb[u] ? atomicAdd(&L.cnt[j[u]], 1u) : 0u;
+                                            bb[u] = L.base[j[u]];
+                                        }
+#pragma unroll
+                                        for (int u = 0; u < 4; u++)
+                                            if (hb[u] && bb[u] != NO_SLOT) a.ids[bb[u] + k2[u]] = oid[u];
                                     }
                                 }
                             }
