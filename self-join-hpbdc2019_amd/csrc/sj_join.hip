@@ -35,6 +35,8 @@ namespace sj {
 constexpr int JB = 256;         // traversal block size (4 waves)
 constexpr int WPB = JB / 64;    // waves per block
 constexpr int DEV_MAXK = 8;     // indexed dims handled by the device stencil
+constexpr int QSLICES = 8;      // work-queue slices (one counter each)
+constexpr int QSTRIDE = 32;     // u32 words between slice counters (128 B)
 enum { PASS_COUNT = 0, PASS_FILL = 1 };
 enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
 
@@ -515,7 +517,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                 if (!(useL || useC || useR)) lo = hi = e0;
                 if (PASS == PASS_COUNT && valid)
                     cand += (useL ? e1 - e0 : 0) + (useC ? e2 - e1 : 0) + (useR ? e3 - e2 : 0);
-                const bool mirLR = anyMirror, mirC = anyMirror && !originRow;
+                const bool mirC = anyMirror && !originRow;
                 for (uint32_t c0 = u0; c0 < u1; c0 += CH) {
                     const uint32_t m = min((uint32_t)CH, u1 - c0);
                     // stage the chunk
@@ -777,13 +779,21 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
         }
     };
     if (a.tileOrder) {
-        // work queue: tiles in descending cost, one per wave per grab
+        // work queue in QSLICES interleaved slices with one counter each:
+        // blocks are dealt to the XCDs round-robin, so a slice is served by
+        // the waves of one XCD.  A wave's first tile is static; the next
+        // grab is issued before the current tile runs, hiding its latency.
+        const uint32_t ns = min(gridDim.x, (uint32_t)QSLICES);
+        const uint32_t sl = blockIdx.x % ns;
+        const uint32_t nbl = (gridDim.x - sl + ns - 1) / ns;  // blocks of this slice
+        uint64_t i = (uint64_t)(blockIdx.x / ns) * WPB + wave;
         for (;;) {
-            uint32_t idx = 0;
-            if (lane == 0) idx = atomicAdd(a.qctr, 1u);
-            idx = __shfl(idx, 0, 64);
-            if (idx >= nTiles) break;
-            run(a.tileOrder[idx]);
+            const uint64_t q = sl + i * ns;
+            if (q >= nTiles) break;
+            uint32_t nx = 0;
+            if (lane == 0) nx = atomicAdd(a.qctr + sl * QSTRIDE, 1u);
+            run(a.tileOrder[q]);
+            i = (uint64_t)nbl * WPB + readlane_u32(nx, 0);
         }
     } else {
         // persistent grid-stride over the tiles in cell order
@@ -875,21 +885,72 @@ __global__ void k_desc_keys(const uint32_t *w, uint64_t n, uint32_t *key, uint32
     val[i] = (uint32_t)i;
 }
 
-// tile cost: candidates of its points (cell workload per point); one wave
-// per tile
-__global__ __launch_bounds__(JB) void k_tile_cost(const uint32_t *w, const uint32_t *cellOf,
-                                                  const uint32_t *tileStart, uint32_t nTiles, uint32_t *key,
-                                                  uint32_t *val) {
-    const uint32_t tile = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (tile >= nTiles) return;
-    const uint64_t s = (uint64_t)tileStart[tile] + __lane_id();
-    uint32_t v = s < tileStart[tile + 1] ? w[cellOf[s]] : 0u;
+// Descending 8-bit log-scale bucket of a cost: 8 steps per octave.
+__device__ __forceinline__ uint32_t cost_key(uint32_t v) {
+    if (v == 0) return 255u;
+    const int e = 31 - __builtin_clz(v);
+    const uint32_t m = e >= 3 ? (v >> (e - 3)) & 7u : (v << (3 - e)) & 7u;
+    return 255u - min(255u, 8u * (uint32_t)e + m);
+}
+
+// Tile cost: the candidates the tile stages, i.e. the union of its
+// neighbour cell ranges over the 3^(k-1) neighbour rows.  Entries from
+// nTiles to bound get the last key, so after a stable sort the first
+// nTiles entries are the tiles, heaviest bucket first and in cell order
+// inside a bucket (neighbouring tiles stay close in the queue).
+template <int K>
+__global__ __launch_bounds__(JB) void k_tile_cost(JoinArgs a, uint32_t bound, uint32_t *key, uint32_t *val) {
+    const uint32_t t = blockIdx.x * JB + threadIdx.x;
+    if (t >= bound) return;
+    uint32_t cost = 0;
+    if (t < *a.nTilesDev) {
+        const uint32_t s0 = a.tileStart[t], s1 = a.tileStart[t + 1] - 1u;
+        const uint32_t cA = a.cellOf[s0], cB = a.cellOf[s1];
+        uint32_t c[K];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (__lane_id() == 0) {
-        key[tile] = 0xFFFFFFFFu - v;
-        val[tile] = tile;
+        for (int j = 0; j < K; j++) c[j] = a.cellCoord[(uint64_t)cA * K + j];
+        const uint32_t xb = a.cellCoord[(uint64_t)cB * K];
+        const uint64_t rowBase = a.cellLinear[cA] - c[0];
+        const uint32_t nC0 = a.nCells[0];
+        const uint32_t ux0 = c[0] >= 1 ? c[0] - 1 : 0u;
+        const uint32_t ux1 = xb + 1 < nC0 ? xb + 1 : nC0 - 1;
+        int o[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) o[j] = -1;
+        int nrow = 1;
+#pragma unroll
+        for (int j = 1; j < K; j++) nrow *= 3;
+        for (int it = 0; it < nrow; ++it) {
+            bool inb = true;
+            int64_t dl = 0;
+#pragma unroll
+            for (int j = 1; j < K; j++) {
+                const int64_t v = (int64_t)c[j] + o[j];
+                inb = inb && v >= 0 && v < (int64_t)a.nCells[j];
+                dl += (int64_t)o[j] * a.mult[j];
+            }
+            if (inb) {
+                const uint64_t rb = (uint64_t)((int64_t)rowBase + dl);
+                cost += pos_lb(a, rb + ux1 + 1) - pos_lb(a, rb + ux0);
+            }
+            bool carry = true;
+#pragma unroll
+            for (int j = 1; j < K; j++) {
+                if (carry) {
+                    if (o[j] == 1) {
+                        o[j] = -1;
+                    } else {
+                        o[j]++;
+                        carry = false;
+                    }
+                }
+            }
+        }
+        key[t] = cost_key(cost);
+    } else {
+        key[t] = 255u;
     }
+    val[t] = t;
 }
 
 // ----------------------------------------------------- small row kernels
@@ -1025,6 +1086,21 @@ static WorkKernel pick_workload(int k) {
     case 6: return k_workload<6>;
     case 7: return k_workload<7>;
     case 8: return k_workload<8>;
+    }
+    return nullptr;
+}
+
+using TileCostKernel = void (*)(JoinArgs, uint32_t, uint32_t *, uint32_t *);
+static TileCostKernel pick_tile_cost(int k) {
+    switch (k) {
+    case 1: return k_tile_cost<1>;
+    case 2: return k_tile_cost<2>;
+    case 3: return k_tile_cost<3>;
+    case 4: return k_tile_cost<4>;
+    case 5: return k_tile_cost<5>;
+    case 6: return k_tile_cost<6>;
+    case 7: return k_tile_cost<7>;
+    case 8: return k_tile_cost<8>;
     }
     return nullptr;
 }
@@ -1174,27 +1250,28 @@ static sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint3
     return SJ_OK;
 }
 
-// Work queue: tiles in descending candidate count.
-static sj_status build_tile_queue(sj_grid_s *g, int lid, const DBuf<uint32_t> &tileStart,
-                                  const DBuf<uint32_t> &nTilesDev, DBuf<uint32_t> &order) {
+// Work queue: tiles by descending staged-candidate bucket.  The tile count
+// stays on the device: the sort covers an upper bound (one tile per 64
+// points plus one per non-empty row, at most one per non-empty cell).
+static sj_status build_tile_queue(sj_grid_s *g, const JoinArgs &a, uint64_t len, DBuf<uint32_t> &order) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
-    uint64_t *pin = g->ctx->pinned;
-    SJ_TRY_HIP(hipMemcpyAsync(pin, nTilesDev.p, 4, hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipStreamSynchronize(s));
-    const uint32_t nTiles = (uint32_t)(pin[0] & 0xffffffffu);
-    DBuf<uint32_t> w, key, key2, val2;
-    SJ_TRY(cell_workload(g, lid, w));
-    SJ_TRY(key.alloc(pool, nTiles));
-    SJ_TRY(key2.alloc(pool, nTiles));
-    SJ_TRY(order.alloc(pool, nTiles));
-    SJ_TRY(val2.alloc(pool, nTiles));
-    if (nTiles == 0) return SJ_OK;
-    hipLaunchKernelGGL(k_tile_cost, dim3((nTiles + WPB - 1) / WPB), dim3(JB), 0, s, w.p, g->cellOf.p,
-                       tileStart.p, nTiles, key.p, order.p);
+    const uint64_t bound64 = (len + 63) / 64 + std::min<uint64_t>(len, g->nNonEmpty);
+    if (bound64 > 0xFFFFFFFFull) {
+        set_error("work queue: %llu tiles exceed 32-bit indices", (unsigned long long)bound64);
+        return SJ_ERR_OVERFLOW;
+    }
+    const uint32_t bound = (uint32_t)bound64;
+    DBuf<uint32_t> key, key2, val2;
+    SJ_TRY(key.alloc(pool, bound));
+    SJ_TRY(key2.alloc(pool, bound));
+    SJ_TRY(order.alloc(pool, bound));
+    SJ_TRY(val2.alloc(pool, bound));
+    if (bound == 0) return SJ_OK;
+    hipLaunchKernelGGL(pick_tile_cost(g->k), dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, key.p, order.p);
     SJ_LAUNCH_CHECK();
     uint32_t *kp = key.p, *ka = key2.p, *vp = order.p, *va = val2.p;
-    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nTiles, 32));
+    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, bound, 8));
     if (vp != order.p) std::swap(order, val2);
     return SJ_OK;
 }
@@ -1353,9 +1430,9 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     a.nTilesDev = nTilesDev.p;
     if (P->schedule == SJ_SCHED_WORKQUEUE) {
         tw.start();
-        SJ_TRY(build_tile_queue(g, P->form == SJ_FORM_LID ? 1 : 0, tileStart, nTilesDev, tileOrder));
+        SJ_TRY(build_tile_queue(g, a, n, tileOrder));
         tw.stop();
-        SJ_TRY(qctr.alloc(pool, 1));
+        SJ_TRY(qctr.alloc(pool, QSLICES * QSTRIDE));
         a.tileOrder = tileOrder.p;
         a.qctr = qctr.p;
     }
@@ -1368,7 +1445,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(stat.alloc(pool, 8));
     SJ_TRY(fill_u32(s, mir.p, n, 0u));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p), 8, 0ull));
-    if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
+    if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
     a.stat = stat.p;
@@ -1414,7 +1491,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         SJ_LAUNCH_CHECK();
         a.ids = r->d_ids.p;
         a.outBase = 0;
-        if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, 1, 0u));
+        if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
         tf.start();
         SJ_TRY(launch_join(g, a, PASS_FILL, prec));
         tf.stop();
