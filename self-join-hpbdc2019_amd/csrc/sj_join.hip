@@ -29,192 +29,9 @@
 #include <cstring>
 
 #include "sj_common.hpp"
+#include "sj_traverse.hpp"
 
 namespace sj {
-
-constexpr int JB = 256;         // traversal block size (4 waves)
-constexpr int WPB = JB / 64;    // waves per block
-constexpr int DEV_MAXK = 8;     // indexed dims handled by the device stencil
-constexpr int QSLICES = 8;      // work-queue slices (one counter each)
-constexpr int QSTRIDE = 32;     // u32 words between slice counters (128 B)
-enum { PASS_COUNT = 0, PASS_FILL = 1 };
-enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
-
-struct JoinArgs {
-    const double *pts;        // n x d, cell order
-    const float *pts32;       // n x d, (x - origin) in FP32, or null
-    const uint32_t *origId;   // original ID of each sorted position
-    const uint32_t *cellOf;
-    const uint32_t *cellStart;
-    const uint64_t *cellLinear;
-    const uint32_t *cellCoord;
-    const int32_t *dense;
-    const uint32_t *posLB;    // dense grids: first position with linear ID >= L
-    uint64_t nne;
-    uint32_t nCells[DEV_MAXK];
-    int64_t mult[DEV_MAXK];
-    int d, k, form, useTiles;
-    double T;
-    float lo32, hi32, T32, pad1;
-    // query range (sorted positions) and tiles of 64 over it
-    uint32_t qBegin, qEnd;
-    const uint32_t *tileStart;  // tile t = sorted positions [tileStart[t], tileStart[t+1])
-    const uint32_t *nTilesDev;  // number of tiles (device)
-    const uint32_t *tileOrder;  // work queue order of tiles, or null
-    uint32_t *qctr;
-    // rows written by this launch (sorted positions)
-    uint32_t wBegin, wEnd;
-    // count pass
-    uint32_t *ownCnt, *mirCnt;
-    unsigned long long *stat;
-    // fill pass: row r of the window starts at offS[r] - outBase
-    const uint64_t *offS;
-    uint64_t outBase;
-    unsigned long long *mirCur;
-    int32_t *ids;
-};
-
-__device__ __forceinline__ uint32_t popc_below64(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Per-lane statistics, reduced across the wave once per kernel: per-lane
-// atomics on one address would serialise in L2.
-struct LaneStats {
-    uint64_t cand = 0, band = 0, mis = 0, err = 0, fallback = 0;
-};
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Bits i in [0,64) with start + i in [A, B).
-__device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint32_t B) {
-    const int64_t lo = max((int64_t)A - (int64_t)start, (int64_t)0);
-    const int64_t hi = min((int64_t)B - (int64_t)start, (int64_t)64);
-    if (hi <= lo) return 0ull;
-    const uint64_t w = (uint64_t)(hi - lo);
-    return (w >= 64 ? ~0ull : ((1ull << w) - 1ull)) << lo;
-}
-
-// Value of v in lane l (l wave-uniform).
-__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
-}
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
-    return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
-}
-
-constexpr unsigned long long NO_SLOT = ~0ull;
-
-__device__ __forceinline__ void flush_stats(const JoinArgs &a, const LaneStats &st) {
-    const uint64_t c = wave_sum_u64(st.cand), b = wave_sum_u64(st.band), m = wave_sum_u64(st.mis);
-    const uint64_t e = wave_sum_u64(st.err), f = wave_sum_u64(st.fallback);
-    if (__lane_id() == 0) {
-        if (c) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)c);
-        if (b) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)b);
-        if (m) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)m);
-        if (e) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)e);
-        if (f) atomicAdd(&a.stat[STAT_FALLBACK], (unsigned long long)f);
-    }
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
-    if (a.dense) return a.dense[lin];
-    uint64_t lo = 0, hi = a.nne;
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
-    }
-    return (lo < a.nne && a.cellLinear[lo] == lin) ? (int64_t)lo : -1;
-}
-
-// first sorted position whose cell has linear ID >= lin
-__device__ __forceinline__ uint32_t pos_lb(const JoinArgs &a, uint64_t lin) {
-    if (a.posLB) return a.posLB[lin];
-    uint64_t lo = 0, hi = a.nne;
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
-    }
-    return a.cellStart[lo];
-}
-
-// Stencil claim rules: FULL every neighbour (kernel.cu:1076-1093); UNICOMP
-// stamp (stamploops.h): the highest differing dimension's coordinate is
-// odd; B-UNICOMP checkerboard (kernel.cu:1346-1392; other k use the
-// balanced rule [off >lex 0] xor ([sum off even] and parity(sum c)));
-// LID: neighbour linear ID larger (kernel.cu:1513-1536).
-template <int K>
-__device__ __forceinline__ bool claims(int form, const uint32_t (&c)[K], const int (&o)[K],
-                                       uint64_t linA, uint64_t linB) {
-    if (form == SJ_FORM_FULL) return true;
-    if (form == SJ_FORM_LID) return linB > linA;
-    if (form == SJ_FORM_UNICOMP) {
-        bool found = false, odd = false;
-#pragma unroll
-        for (int j = K - 1; j >= 0; j--) {
-            if (!found && o[j] != 0) {
-                odd = (c[j] & 1u) != 0;
-                found = true;
-            }
-        }
-        return odd;
-    }
-    uint32_t sc = 0;
-#pragma unroll
-    for (int j = 0; j < K; j++) sc += c[j];
-    const bool odd = sc & 1u;
-    if constexpr (K == 2) {
-        if (odd) return o[1] >= 0 && o[0] != 0;
-        return o[1] > 0 || (o[1] < 0 && o[0] == 0);
-    } else if constexpr (K == 3) {
-        if (odd)
-            return (o[0] != 0 && o[1] >= 0 && o[2] >= 0) || (o[0] == 0 && o[1] >= 0 && o[2] > 0) ||
-                   (o[1] > 0 && o[2] < 0);
-        return (o[1] > 0 && o[2] >= 0) || (o[0] == 0 && o[1] < 0 && o[2] >= 0) ||
-               (o[0] != 0 && o[1] > 0 && o[2] < 0) || (o[1] == 0 && o[2] > 0);
-    } else {
-        int so = 0;
-        bool pos = false, found = false;
-#pragma unroll
-        for (int j = K - 1; j >= 0; j--) {
-            so += o[j];
-            if (!found && o[j] != 0) {
-                pos = o[j] > 0;
-                found = true;
-            }
-        }
-        const bool even = (so & 1) == 0;
-        return pos != (even && odd);
-    }
-}
-
-// FP64 canonical predicate: dimension order, explicit fma.
-template <int D>
-__device__ __forceinline__ double d2_fp64(const double *p, const double *q, int d) {
-    double acc = 0.0;
-    if constexpr (D > 0) {
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            const double df = q[j] - p[j];
-            acc = j == 0 ? df * df : fma(df, df, acc);
-        }
-    } else {
-        for (int j = 0; j < d; j++) {
-            const double df = q[j] - p[j];
-            acc = j == 0 ? df * df : fma(df, df, acc);
-        }
-    }
-    return acc;
-}
 
 template <int D>
 __device__ __forceinline__ float d2_fp32(const float *p, const float *q, int d) {
@@ -968,9 +785,10 @@ __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_
     mirCur[s] = offS[s] - outBase + own[s];
 }
 
-// FP32 copy of the coordinates relative to the grid origin, and the
-// largest |x - origin| (one atomic per block)
-__global__ __launch_bounds__(JB) void k_to_fp32(const double *pts, uint64_t n, int d, int k,
+// FP32 copy of the coordinates relative to the grid origin, rows of
+// `stride` floats (zero padded), and the largest |x - origin| (one atomic
+// per block)
+__global__ __launch_bounds__(JB) void k_to_fp32(const double *pts, uint64_t n, int d, int k, int stride,
                                                 const GridParams *gp, float *out, unsigned int *rmaxBits) {
     __shared__ float wmax[WPB];
     const uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
@@ -979,9 +797,10 @@ __global__ __launch_bounds__(JB) void k_to_fp32(const double *pts, uint64_t n, i
         for (int j = 0; j < d; j++) {
             const double o = j < k ? gp->minArr[j] : 0.0;
             const double v = pts[s * d + j] - o;
-            out[s * d + j] = (float)v;
+            out[s * stride + j] = (float)v;
             r = fmaxf(r, (float)fabs(v));
         }
+        for (int j = d; j < stride; j++) out[s * stride + j] = 0.0f;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off, 64));
@@ -1303,8 +1122,9 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
     return SJ_OK;
 }
 
-static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec) {
+static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, const MfmaConsts *mc) {
     hipStream_t s = g->ctx->stream;
+    if (mc && prec == SJ_PREC_FP32_RECHECK) return launch_join_mfma(a, *mc, pass, g->ctx->cu_count, s);
     JoinKernel kfn = pick_kernel(g->k, g->d, pass, prec);
     if (!kfn) {
         set_error("no traversal kernel for k=%d", g->k);
@@ -1396,13 +1216,18 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 
     // precision form: FP32 copy relative to the grid origin + margin
     int prec = P->precision;
+    MfmaConsts mc;
+    const MfmaConsts *mcp = nullptr;
     if (prec == SJ_PREC_FP32_RECHECK) {
         DBuf<unsigned int> rbits;
         SJ_TRY(rbits.alloc(pool, 1));
         SJ_TRY(fill_u32(s, rbits.p, 1, 0u));
-        if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)g->d));
-        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d,
-                           g->k, g->d_gp.p, g->sortedPts32.p, rbits.p);
+        // d in the matrix-core range: rows padded for the MFMA operands
+        const int DP = a.useTiles ? mfma_padded_dims(g->d) : 0;
+        const int stride = DP ? DP : g->d;
+        if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)stride));
+        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d, g->k, stride,
+                           g->d_gp.p, g->sortedPts32.p, rbits.p);
         SJ_LAUNCH_CHECK();
         uint64_t *pin = ctx->pinned;
         SJ_TRY_HIP(hipMemcpyAsync(pin, rbits.p, 4, hipMemcpyDeviceToHost, s));
@@ -1413,7 +1238,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         // the FP32 rounding of |v| may be below |v|: widen by one FP32 ulp
         const double R = (double)std::nextafter(rf, INFINITY);
         Fp32Margin m = fp32_margin(T, R, g->d);
-        if (m.usable) {
+        bool mok = false;
+        if (DP) mc = mfma_consts(T, R, g->d, DP, &mok);
+        if (DP && mok) {
+            a.pts32 = g->sortedPts32.p;
+            mcp = &mc;
+        } else if (DP) {
+            prec = SJ_PREC_FP64;  // FP32 copy would not bound the error usefully
+        } else if (m.usable) {
             a.pts32 = g->sortedPts32.p;
             a.lo32 = m.lo;
             a.hi32 = m.hi;
@@ -1450,7 +1282,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     a.mirCnt = mir.p;
     a.stat = stat.p;
     tc.start();
-    SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
+    SJ_TRY(launch_join(g, a, PASS_COUNT, prec, mcp));
     tc.stop();
 
     // row offsets in cell order
@@ -1473,6 +1305,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     r->stats.fp32_band_pairs = pin[1 + STAT_BAND];
     r->stats.fp32_misclassified = pin[1 + STAT_MIS];
     r->stats.overflow_retries = 0;
+    if (pin[1 + STAT_ERR]) {
+        set_error("join: count pass met %llu inconsistent tiles", (unsigned long long)pin[1 + STAT_ERR]);
+        return SJ_ERR_STATE;
+    }
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
     SJ_TRY(r->d_rowIds.alloc(pool, n));
     SJ_TRY_HIP(hipMemcpyAsync(r->d_rowIds.p, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
@@ -1493,7 +1329,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         a.outBase = 0;
         if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
         tf.start();
-        SJ_TRY(launch_join(g, a, PASS_FILL, prec));
+        SJ_TRY(launch_join(g, a, PASS_FILL, prec, mcp));
         tf.stop();
         if (P->sort_rows) {
             tr.start();
@@ -1571,7 +1407,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             ab.wEnd = s1;
             ab.outBase = base;
             tf.start();
-            SJ_TRY(launch_join(g, ab, PASS_FILL, prec));
+            SJ_TRY(launch_join(g, ab, PASS_FILL, prec, mcp));
             tf.stop();
             if (P->sort_rows) SJ_TRY(sort_rows_device(ctx, r->d_offsets.p + s0, dbuf.p, s1 - s0, base));
             const uint64_t cnt = hOff[s1] - base;

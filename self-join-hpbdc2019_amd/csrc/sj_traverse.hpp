@@ -1,0 +1,211 @@
+// Device-side definitions shared by the traversal kernels: launch
+// constants, the argument block, lane/wave helpers, cell lookups, the
+// stencil claim rules and the canonical FP64 predicate.
+#pragma once
+
+#include "sj_common.hpp"
+
+namespace sj {
+
+constexpr int JB = 256;         // traversal block size (4 waves)
+constexpr int WPB = JB / 64;    // waves per block
+constexpr int DEV_MAXK = 8;     // indexed dims handled by the device stencil
+constexpr int QSLICES = 8;      // work-queue slices (one counter each)
+constexpr int QSTRIDE = 32;     // u32 words between slice counters (128 B)
+enum { PASS_COUNT = 0, PASS_FILL = 1 };
+enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
+
+struct JoinArgs {
+    const double *pts;        // n x d, cell order
+    const float *pts32;       // n x d, (x - origin) in FP32, or null
+    const uint32_t *origId;   // original ID of each sorted position
+    const uint32_t *cellOf;
+    const uint32_t *cellStart;
+    const uint64_t *cellLinear;
+    const uint32_t *cellCoord;
+    const int32_t *dense;
+    const uint32_t *posLB;    // dense grids: first position with linear ID >= L
+    uint64_t nne;
+    uint32_t nCells[DEV_MAXK];
+    int64_t mult[DEV_MAXK];
+    int d, k, form, useTiles;
+    double T;
+    float lo32, hi32, T32, pad1;
+    // query range (sorted positions) and tiles of 64 over it
+    uint32_t qBegin, qEnd;
+    const uint32_t *tileStart;  // tile t = sorted positions [tileStart[t], tileStart[t+1])
+    const uint32_t *nTilesDev;  // number of tiles (device)
+    const uint32_t *tileOrder;  // work queue order of tiles, or null
+    uint32_t *qctr;
+    // rows written by this launch (sorted positions)
+    uint32_t wBegin, wEnd;
+    // count pass
+    uint32_t *ownCnt, *mirCnt;
+    unsigned long long *stat;
+    // fill pass: row r of the window starts at offS[r] - outBase
+    const uint64_t *offS;
+    uint64_t outBase;
+    unsigned long long *mirCur;
+    int32_t *ids;
+};
+
+__device__ __forceinline__ uint32_t popc_below64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Per-lane statistics, reduced across the wave once per kernel: per-lane
+// atomics on one address would serialise in L2.
+struct LaneStats {
+    uint64_t cand = 0, band = 0, mis = 0, err = 0, fallback = 0;
+};
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Bits i in [0,64) with start + i in [A, B).
+__device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint32_t B) {
+    const int64_t lo = max((int64_t)A - (int64_t)start, (int64_t)0);
+    const int64_t hi = min((int64_t)B - (int64_t)start, (int64_t)64);
+    if (hi <= lo) return 0ull;
+    const uint64_t w = (uint64_t)(hi - lo);
+    return (w >= 64 ? ~0ull : ((1ull << w) - 1ull)) << lo;
+}
+
+// Value of v in lane l (l wave-uniform).
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+    return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
+}
+
+constexpr unsigned long long NO_SLOT = ~0ull;
+
+__device__ __forceinline__ void flush_stats(const JoinArgs &a, const LaneStats &st) {
+    const uint64_t c = wave_sum_u64(st.cand), b = wave_sum_u64(st.band), m = wave_sum_u64(st.mis);
+    const uint64_t e = wave_sum_u64(st.err), f = wave_sum_u64(st.fallback);
+    if (__lane_id() == 0) {
+        if (c) atomicAdd(&a.stat[STAT_CAND], (unsigned long long)c);
+        if (b) atomicAdd(&a.stat[STAT_BAND], (unsigned long long)b);
+        if (m) atomicAdd(&a.stat[STAT_MIS], (unsigned long long)m);
+        if (e) atomicAdd(&a.stat[STAT_ERR], (unsigned long long)e);
+        if (f) atomicAdd(&a.stat[STAT_FALLBACK], (unsigned long long)f);
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
+    if (a.dense) return a.dense[lin];
+    uint64_t lo = 0, hi = a.nne;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
+    }
+    return (lo < a.nne && a.cellLinear[lo] == lin) ? (int64_t)lo : -1;
+}
+
+// first sorted position whose cell has linear ID >= lin
+__device__ __forceinline__ uint32_t pos_lb(const JoinArgs &a, uint64_t lin) {
+    if (a.posLB) return a.posLB[lin];
+    uint64_t lo = 0, hi = a.nne;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
+    }
+    return a.cellStart[lo];
+}
+
+// Stencil claim rules: FULL every neighbour (kernel.cu:1076-1093); UNICOMP
+// stamp (stamploops.h): the highest differing dimension's coordinate is
+// odd; B-UNICOMP checkerboard (kernel.cu:1346-1392; other k use the
+// balanced rule [off >lex 0] xor ([sum off even] and parity(sum c)));
+// LID: neighbour linear ID larger (kernel.cu:1513-1536).
+template <int K>
+__device__ __forceinline__ bool claims(int form, const uint32_t (&c)[K], const int (&o)[K],
+                                       uint64_t linA, uint64_t linB) {
+    if (form == SJ_FORM_FULL) return true;
+    if (form == SJ_FORM_LID) return linB > linA;
+    if (form == SJ_FORM_UNICOMP) {
+        bool found = false, odd = false;
+#pragma unroll
+        for (int j = K - 1; j >= 0; j--) {
+            if (!found && o[j] != 0) {
+                odd = (c[j] & 1u) != 0;
+                found = true;
+            }
+        }
+        return odd;
+    }
+    uint32_t sc = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) sc += c[j];
+    const bool odd = sc & 1u;
+    if constexpr (K == 2) {
+        if (odd) return o[1] >= 0 && o[0] != 0;
+        return o[1] > 0 || (o[1] < 0 && o[0] == 0);
+    } else if constexpr (K == 3) {
+        if (odd)
+            return (o[0] != 0 && o[1] >= 0 && o[2] >= 0) || (o[0] == 0 && o[1] >= 0 && o[2] > 0) ||
+                   (o[1] > 0 && o[2] < 0);
+        return (o[1] > 0 && o[2] >= 0) || (o[0] == 0 && o[1] < 0 && o[2] >= 0) ||
+               (o[0] != 0 && o[1] > 0 && o[2] < 0) || (o[1] == 0 && o[2] > 0);
+    } else {
+        int so = 0;
+        bool pos = false, found = false;
+#pragma unroll
+        for (int j = K - 1; j >= 0; j--) {
+            so += o[j];
+            if (!found && o[j] != 0) {
+                pos = o[j] > 0;
+                found = true;
+            }
+        }
+        const bool even = (so & 1) == 0;
+        return pos != (even && odd);
+    }
+}
+
+// FP64 canonical predicate: dimension order, explicit fma.
+template <int D>
+__device__ __forceinline__ double d2_fp64(const double *p, const double *q, int d) {
+    double acc = 0.0;
+    if constexpr (D > 0) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            const double df = q[j] - p[j];
+            acc = j == 0 ? df * df : fma(df, df, acc);
+        }
+    } else {
+        for (int j = 0; j < d; j++) {
+            const double df = q[j] - p[j];
+            acc = j == 0 ? df * df : fma(df, df, acc);
+        }
+    }
+    return acc;
+}
+
+// ------------------------------------------------- matrix-core path
+constexpr int MFMA_MIN_D = 16;  // FP32+recheck joins with MFMA_MIN_D <= d <= MFMA_MAX_D
+constexpr int MFMA_MAX_D = 64;  // run on the matrix cores
+
+// Decision constants of the matrix-core distance (see mfma_consts).
+struct MfmaConsts {
+    float alpha, beta;  // error bound alpha * (|q|^2 + |c|^2) + beta
+    float tlo, thi;     // certain hit at or below tlo, certain miss above thi
+    float t32;          // FP32 threshold: the decision before recheck
+    int dp;             // padded dims = row stride of the FP32 copy
+};
+
+// Padded dims of the FP32 copy for the matrix-core path, 0 if d is outside it.
+int mfma_padded_dims(int d);
+MfmaConsts mfma_consts(double T, double Rcoord, int d, int DP, bool *usable);
+sj_status launch_join_mfma(const JoinArgs &a, const MfmaConsts &mc, int pass, int cu_count, hipStream_t s);
+
+}  // namespace sj
