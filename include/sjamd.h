@@ -190,6 +190,52 @@ SJ_API sj_status sj_import_csv(const char *path, int32_t d, double **points, uin
 SJ_API sj_status sj_sort_in_nd_bins(double *points, uint64_t n, int32_t d, int32_t stable);
 SJ_API void sj_free_host(void *p);
 
+/* ---- Many joins in one graph ------------------------------------------
+ * No counterpart in the reference, which runs one join per process
+ * (main.cu:36-226): a batch of independent self-joins of different size,
+ * dimension and eps, joined by one persistent kernel per pass over all
+ * instances' tiles and recorded once into a hipGraph. */
+typedef struct sj_instance {
+    const double *points;  /* device, n x d float64, row-major */
+    uint64_t n;
+    int32_t d, k;          /* k indexed dims, 1..8 */
+    double eps;
+} sj_instance;
+
+typedef struct sj_many_tuning {
+    int32_t form;          /* sj_form */
+    int32_t schedule;      /* sj_schedule */
+    int32_t blocks_per_cu; /* persistent blocks of 256 threads per CU (0: 8) */
+    int32_t queue_chunk;   /* work-queue tiles per grab (0: 1) */
+} sj_many_tuning;
+
+typedef struct sj_many_s *sj_many;
+
+/* Plan: builds every grid and tile list, sizes the shared neighbour-ID
+ * array with one count run, records the graph and runs it once.  params:
+ * form, precision, schedule, queue_chunk; capacity must be 0. */
+SJ_API sj_status sj_many_create(sj_context ctx, const sj_instance *instances, int32_t count,
+                                const sj_join_params *params, sj_many *out);
+/* One graph launch: count, offsets and fill of every instance. */
+SJ_API sj_status sj_many_run(sj_many m);
+/* Times graph replays over stencil form (when tune_form != 0), schedule,
+ * blocks per CU and queue chunk; keeps and returns the fastest. */
+SJ_API sj_status sj_many_autotune(sj_many m, int32_t tune_form, int32_t reps, sj_many_tuning *chosen);
+SJ_API sj_status sj_many_set_tuning(sj_many m, const sj_many_tuning *tuning);
+SJ_API sj_status sj_many_get_tuning(sj_many m, sj_many_tuning *tuning);
+/* Device neighbour table of instance i, as sj_result_device_table:
+ * offsets[n_i + 1] relative to ids, rows in cell order, rowIds the
+ * original point ID of each row. */
+SJ_API sj_status sj_many_instance(sj_many m, int32_t i, const uint64_t **d_offsets, const int32_t **d_ids,
+                                  const uint32_t **d_rowIds, uint64_t *total);
+/* Counters of instance i, or of the whole batch for i = -1. */
+SJ_API sj_status sj_many_stats(sj_many m, int32_t i, sj_stats *stats);
+/* Wall time of the plan, of its grid builds, and device time of the last
+ * graph replay, in ms. */
+SJ_API sj_status sj_many_times(sj_many m, double *ms_plan, double *ms_grid, double *ms_run);
+SJ_API int32_t sj_many_count(sj_many m);
+SJ_API void sj_many_free(sj_many m);
+
 /* Eps threshold T = max{x : sqrt_rn(x) <= eps}; sqrt(d2) <= eps iff d2 <= T. */
 SJ_API double sj_eps_threshold(double eps);
 

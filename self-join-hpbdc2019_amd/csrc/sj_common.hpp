@@ -114,6 +114,9 @@ struct HostStage {
 };
 
 // ------------------------------------------------------------- launches
+uint64_t scan_temp_count(uint64_t n);
+sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint64_t n, uint64_t *out,
+                                        uint64_t *sums);
 sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
                                     uint64_t *out /* n+1 */);
 sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,

@@ -619,6 +619,93 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
     flush_stats(a, st);
 }
 
+// ------------------------------------------------------------ many joins
+// One persistent launch over the tiles of many independent joins: work
+// item w is tile workTile[w] of instance workInst[w], whose arguments are
+// args[instance].  Each tile runs the traversal specialised for its
+// instance's (k, d, precision); statistics go to the instance's counters.
+template <int A, int B>
+struct cmax {
+    static constexpr int v = A > B ? A : B;
+};
+template <int D>
+struct LdsMax {
+    static constexpr int v = cmax<cmax<(int)sizeof(TileLds<D, 0>), (int)sizeof(TileLds<D, 1>)>::v,
+                                  LdsMax<D - 1>::v>::v;
+};
+template <>
+struct LdsMax<0> {
+    static constexpr int v = 0;
+};
+constexpr int MANY_LDS = (LdsMax<DEV_MAXK>::v + 15) / 16 * 16;
+
+template <int K, int PASS>
+__device__ __forceinline__ void many_tile(const JoinArgs &a, uint32_t tile, unsigned char *raw, LaneStats &st) {
+    const bool fp32 = a.pts32 != nullptr;
+    if (a.d == K && a.useTiles) {
+        if (fp32) process_tile<K, K, PASS, 1>(a, tile, *reinterpret_cast<TileLds<K, 1> *>(raw), st);
+        else process_tile<K, K, PASS, 0>(a, tile, *reinterpret_cast<TileLds<K, 0> *>(raw), st);
+    } else {
+        const uint64_t s = (uint64_t)a.tileStart[tile] + __lane_id();
+        if (s < a.tileStart[tile + 1]) {
+            if (fp32) process_point<K, 0, PASS, 1>(a, (uint32_t)s, st);
+            else process_point<K, 0, PASS, 0>(a, (uint32_t)s, st);
+        }
+    }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(JB) void k_join_many(const JoinArgs *__restrict__ args,
+                                                  const uint32_t *__restrict__ workInst,
+                                                  const uint32_t *__restrict__ workTile, uint32_t W,
+                                                  uint32_t *qctr, uint32_t chunk, int useQueue) {
+    __shared__ __align__(16) unsigned char lds[WPB][MANY_LDS];
+    const unsigned wave = threadIdx.x >> 6;
+    const unsigned lane = __lane_id();
+    unsigned char *raw = lds[wave];
+    LaneStats st;
+    uint32_t cur = 0xFFFFFFFFu;
+    auto run = [&](uint32_t w) __attribute__((always_inline)) {
+        const uint32_t inst = (uint32_t)__builtin_amdgcn_readfirstlane((int)workInst[w]);
+        const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)workTile[w]);
+        if (inst != cur) {
+            if (cur != 0xFFFFFFFFu) flush_stats(args[cur], st);
+            st = LaneStats();
+            cur = inst;
+        }
+        const JoinArgs &a = args[inst];
+        switch (a.k) {
+        case 1: many_tile<1, PASS>(a, tile, raw, st); break;
+        case 2: many_tile<2, PASS>(a, tile, raw, st); break;
+        case 3: many_tile<3, PASS>(a, tile, raw, st); break;
+        case 4: many_tile<4, PASS>(a, tile, raw, st); break;
+        case 5: many_tile<5, PASS>(a, tile, raw, st); break;
+        case 6: many_tile<6, PASS>(a, tile, raw, st); break;
+        case 7: many_tile<7, PASS>(a, tile, raw, st); break;
+        case 8: many_tile<8, PASS>(a, tile, raw, st); break;
+        }
+    };
+    if (useQueue) {
+        // chunks of `chunk` consecutive items: the wave's first chunk is
+        // static, later ones come from the counter; the next grab is issued
+        // before the current chunk runs
+        const uint64_t nw = (uint64_t)gridDim.x * WPB;
+        uint64_t m = (uint64_t)blockIdx.x * WPB + wave;
+        for (;;) {
+            const uint64_t b0 = m * chunk;
+            if (b0 >= W) break;
+            uint32_t nx = 0;
+            if (lane == 0) nx = atomicAdd(qctr, 1u);
+            const uint64_t b1 = min<uint64_t>(b0 + chunk, W);
+            for (uint64_t w = b0; w < b1; w++) run((uint32_t)w);
+            m = nw + readlane_u32(nx, 0);
+        }
+    } else {
+        for (uint32_t w = blockIdx.x * WPB + wave; w < W; w += gridDim.x * WPB) run(w);
+    }
+    if (cur != 0xFFFFFFFFu) flush_stats(args[cur], st);
+}
+
 // Tiles: runs of at most 64 consecutive points of one cell row within
 // [q0, q1).  A head starts every row and every 64 points inside a row.
 __global__ __launch_bounds__(JB) void k_tile_heads(JoinArgs a, uint32_t q0, uint32_t q1, uint32_t *head) {
@@ -1006,7 +1093,7 @@ struct EvTimer {
     }
 };
 
-static JoinArgs base_args(sj_grid_s *g, int form, double T) {
+JoinArgs base_args(sj_grid_s *g, int form, double T) {
     JoinArgs a;
     memset(&a, 0, sizeof a);
     a.pts = g->sortedPts.p;
@@ -1046,7 +1133,7 @@ static sj_status cell_workload(sj_grid_s *g, int lid, DBuf<uint32_t> &w) {
 }
 
 // Row-aligned tiles over [q0, q1); the count stays on the device.
-static sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
+sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
                              DBuf<uint32_t> &nTiles) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
@@ -1140,6 +1227,34 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
     return SJ_OK;
 }
 
+sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
+                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, unsigned blocks,
+                           hipStream_t s) {
+    if (W == 0) return SJ_OK;
+    if (chunk == 0) chunk = 1;
+    if (blocks == 0) blocks = 1;
+    if (pass == PASS_COUNT)
+        hipLaunchKernelGGL(k_join_many<PASS_COUNT>, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W,
+                           qctr, chunk, useQueue);
+    else
+        hipLaunchKernelGGL(k_join_many<PASS_FILL>, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W,
+                           qctr, chunk, useQueue);
+    SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
+sj_status launch_tile_cost(const JoinArgs &a, int k, uint32_t bound, uint32_t *key, uint32_t *val, hipStream_t s) {
+    TileCostKernel kfn = pick_tile_cost(k);
+    if (!kfn) {
+        set_error("tile cost: k=%d above the device stencil limit", k);
+        return SJ_ERR_UNSUPPORTED;
+    }
+    if (bound == 0) return SJ_OK;
+    hipLaunchKernelGGL(kfn, dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, key, val);
+    SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
 // Sampled estimate of |S| (every stride-th original point, full stencil).
 sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
     Pool *pool = &g->ctx->pool;
@@ -1183,6 +1298,52 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
 
 sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base);
 
+// FP32 copy relative to the grid origin and the decision constants of the
+// precision form.  Falls back to FP64 (*prec) when the FP32 error bound
+// would not be useful.  One host sync (the coordinate bound).
+sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec, MfmaConsts *mc, const MfmaConsts **mcp) {
+    sj_context ctx = g->ctx;
+    Pool *pool = &ctx->pool;
+    hipStream_t s = ctx->stream;
+    const uint64_t n = g->n;
+    *mcp = nullptr;
+    DBuf<unsigned int> rbits;
+    SJ_TRY(rbits.alloc(pool, 1));
+    SJ_TRY(fill_u32(s, rbits.p, 1, 0u));
+    // d in the matrix-core range: rows padded for the MFMA operands
+    const int DP = a.useTiles ? mfma_padded_dims(g->d) : 0;
+    const int stride = DP ? DP : g->d;
+    if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)stride));
+    hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d, g->k, stride,
+                       g->d_gp.p, g->sortedPts32.p, rbits.p);
+    SJ_LAUNCH_CHECK();
+    uint64_t *pin = ctx->pinned;
+    SJ_TRY_HIP(hipMemcpyAsync(pin, rbits.p, 4, hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipStreamSynchronize(s));
+    const uint32_t hb = (uint32_t)(pin[0] & 0xffffffffu);
+    float rf;
+    memcpy(&rf, &hb, 4);
+    // the FP32 rounding of |v| may be below |v|: widen by one FP32 ulp
+    const double R = (double)std::nextafter(rf, INFINITY);
+    Fp32Margin m = fp32_margin(a.T, R, g->d);
+    bool mok = false;
+    if (DP) *mc = mfma_consts(a.T, R, g->d, DP, &mok);
+    if (DP && mok) {
+        a.pts32 = g->sortedPts32.p;
+        *mcp = mc;
+    } else if (DP) {
+        *prec = SJ_PREC_FP64;  // FP32 copy would not bound the error usefully
+    } else if (m.usable) {
+        a.pts32 = g->sortedPts32.p;
+        a.lo32 = m.lo;
+        a.hi32 = m.hi;
+        a.T32 = m.t32;
+    } else {
+        *prec = SJ_PREC_FP64;  // band would cover the whole range
+    }
+    return SJ_OK;
+}
+
 sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     if (!g || !P || !r) {
@@ -1218,42 +1379,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     int prec = P->precision;
     MfmaConsts mc;
     const MfmaConsts *mcp = nullptr;
-    if (prec == SJ_PREC_FP32_RECHECK) {
-        DBuf<unsigned int> rbits;
-        SJ_TRY(rbits.alloc(pool, 1));
-        SJ_TRY(fill_u32(s, rbits.p, 1, 0u));
-        // d in the matrix-core range: rows padded for the MFMA operands
-        const int DP = a.useTiles ? mfma_padded_dims(g->d) : 0;
-        const int stride = DP ? DP : g->d;
-        if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)stride));
-        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d, g->k, stride,
-                           g->d_gp.p, g->sortedPts32.p, rbits.p);
-        SJ_LAUNCH_CHECK();
-        uint64_t *pin = ctx->pinned;
-        SJ_TRY_HIP(hipMemcpyAsync(pin, rbits.p, 4, hipMemcpyDeviceToHost, s));
-        SJ_TRY_HIP(hipStreamSynchronize(s));
-        const uint32_t hb = (uint32_t)(pin[0] & 0xffffffffu);
-        float rf;
-        memcpy(&rf, &hb, 4);
-        // the FP32 rounding of |v| may be below |v|: widen by one FP32 ulp
-        const double R = (double)std::nextafter(rf, INFINITY);
-        Fp32Margin m = fp32_margin(T, R, g->d);
-        bool mok = false;
-        if (DP) mc = mfma_consts(T, R, g->d, DP, &mok);
-        if (DP && mok) {
-            a.pts32 = g->sortedPts32.p;
-            mcp = &mc;
-        } else if (DP) {
-            prec = SJ_PREC_FP64;  // FP32 copy would not bound the error usefully
-        } else if (m.usable) {
-            a.pts32 = g->sortedPts32.p;
-            a.lo32 = m.lo;
-            a.hi32 = m.hi;
-            a.T32 = m.t32;
-        } else {
-            prec = SJ_PREC_FP64;  // band would cover the whole range
-        }
-    }
+    if (prec == SJ_PREC_FP32_RECHECK) SJ_TRY(prepare_fp32(g, a, &prec, &mc, &mcp));
 
     // row-aligned tiles; work queue of tiles in descending workload
     DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr;

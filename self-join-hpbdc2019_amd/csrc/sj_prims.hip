@@ -255,6 +255,29 @@ sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *i
     return scan_exclusive<uint32_t, uint64_t>(pool, s, in, n, out);
 }
 
+uint64_t scan_temp_count(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// Same scan with a caller-owned block-sum buffer of scan_temp_count(n)
+// entries: allocates nothing, so it can be recorded into a graph.
+sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint64_t n, uint64_t *out,
+                                        uint64_t *sums) {
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill<uint64_t>, dim3(1), dim3(64), 0, s, out, (uint64_t)1, (uint64_t)0);
+        SJ_LAUNCH_CHECK();
+        return SJ_OK;
+    }
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL((k_scan_reduce<uint32_t, uint64_t>), dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n,
+                       sums);
+    SJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_scan_sums<uint64_t>), dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb);
+    SJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_scan_apply<uint32_t, uint64_t>), dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n,
+                       sums, out, nb);
+    SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
 sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
                                     uint32_t *out) {
     return scan_exclusive<uint32_t, uint32_t>(pool, s, in, n, out);

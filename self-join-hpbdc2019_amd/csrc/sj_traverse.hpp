@@ -208,4 +208,20 @@ int mfma_padded_dims(int d);
 MfmaConsts mfma_consts(double T, double Rcoord, int d, int DP, bool *usable);
 sj_status launch_join_mfma(const JoinArgs &a, const MfmaConsts &mc, int pass, int cu_count, hipStream_t s);
 
+// ------------------------------------------------- host helpers (sj_join.hip)
+double eps_threshold(double eps);
+JoinArgs base_args(sj_grid_s *g, int form, double T);
+sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
+                      DBuf<uint32_t> &nTiles);
+sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec, MfmaConsts *mc, const MfmaConsts **mcp);
+
+// One persistent launch over (instance, tile) work items of many joins.
+sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
+                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, unsigned blocks,
+                           hipStream_t s);
+
+// Work-queue keys of tiles [0, bound) of one join (descending 8-bit
+// log-scale bucket of the staged candidates; val[t] = t).
+sj_status launch_tile_cost(const JoinArgs &a, int k, uint32_t bound, uint32_t *key, uint32_t *val, hipStream_t s);
+
 }  // namespace sj

@@ -15,6 +15,7 @@ Reference interface map (benoitgallet/self-join-hpbdc2019):
   Grid.batch_estimate callGPUBatchEst[WorkQueue] GPU.cu:854-1284
   import_csv         importNDDataset           import_dataset.cpp:34
   sort_in_nd_bins    sortInNDBins              import_dataset.cpp:76
+  Context.many       (no counterpart: many joins in one graph launch)
 """
 import ctypes as C
 import os
@@ -80,6 +81,19 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Instance(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("n", C.c_uint64), ("d", C.c_int32), ("k", C.c_int32),
+                ("eps", C.c_double)]
+
+
+class ManyTuning(C.Structure):
+    _fields_ = [("form", C.c_int32), ("schedule", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("queue_chunk", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class NeighborLookup(C.Structure):
     _fields_ = [("pointID", C.c_int32), ("indexmin", C.c_int32), ("indexmax", C.c_int32),
                 ("batch", C.c_int32)]
@@ -92,6 +106,8 @@ EXPORTS = [
     "sj_result_num_points", "sj_result_device_table", "sj_result_device_csr", "sj_result_copy_csr", "sj_result_copy_pairs",
     "sj_result_neighbor_table", "sj_result_batch_data", "sj_result_free", "sj_workload",
     "sj_batch_estimate", "sj_import_csv", "sj_sort_in_nd_bins", "sj_free_host", "sj_eps_threshold",
+    "sj_many_create", "sj_many_run", "sj_many_autotune", "sj_many_set_tuning", "sj_many_get_tuning",
+    "sj_many_instance", "sj_many_stats", "sj_many_times", "sj_many_count", "sj_many_free",
 ]
 
 _lib = None
@@ -141,6 +157,18 @@ def lib():
     L.sj_sort_in_nd_bins.argtypes = [P(C.c_double), u64, i32, i32]
     L.sj_free_host.argtypes = [vp]
     L.sj_free_host.restype = None
+    L.sj_many_create.argtypes = [vp, P(Instance), i32, P(JoinParams), P(vp)]
+    L.sj_many_run.argtypes = [vp]
+    L.sj_many_autotune.argtypes = [vp, i32, i32, P(ManyTuning)]
+    L.sj_many_set_tuning.argtypes = [vp, P(ManyTuning)]
+    L.sj_many_get_tuning.argtypes = [vp, P(ManyTuning)]
+    L.sj_many_instance.argtypes = [vp, i32, P(vp), P(vp), P(vp), P(u64)]
+    L.sj_many_stats.argtypes = [vp, i32, P(Stats)]
+    L.sj_many_times.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.sj_many_count.argtypes = [vp]
+    L.sj_many_count.restype = i32
+    L.sj_many_free.argtypes = [vp]
+    L.sj_many_free.restype = None
     _lib = L
     return L
 
@@ -243,6 +271,10 @@ class Context:
         _check(lib().sj_self_join(self.h, C.c_void_p(t.data_ptr()), n, d, k, float(eps),
                                   C.byref(p), C.byref(h)))
         return Result(self, h)
+
+    def many(self, instances, **kw):
+        """Plan many joins; instances: iterable of (points, k, eps)."""
+        return Many(self, instances, **kw)
 
 
 class Grid:
@@ -383,3 +415,91 @@ def sort_in_nd_bins(points, stable=False):
     _check(lib().sj_sort_in_nd_bins(p.ctypes.data_as(C.POINTER(C.c_double)), p.shape[0],
                                     p.shape[1], 1 if stable else 0))
     return p
+
+
+class Many:
+    """Many independent self-joins in one persistent, graph-recorded launch
+    per pass (sj_many_* in include/sjamd.h)."""
+
+    def __init__(self, ctx, instances, **kw):
+        self.ctx = ctx
+        self._t = [ctx.to_device(p) for p, _, _ in instances]
+        arr = (Instance * len(self._t))()
+        self.shapes = []
+        for i, (t, (_, k, eps)) in enumerate(zip(self._t, instances)):
+            n, d = t.shape
+            arr[i] = Instance(C.c_void_p(t.data_ptr()), n, d, k, float(eps))
+            self.shapes.append((n, d, k, float(eps)))
+        p = make_params(**kw)
+        h = C.c_void_p()
+        _check(lib().sj_many_create(ctx.h, arr, len(self._t), C.byref(p), C.byref(h)))
+        self.h = h
+
+    def free(self):
+        if self.h:
+            lib().sj_many_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def run(self):
+        _check(lib().sj_many_run(self.h))
+
+    def autotune(self, tune_form=False, reps=3):
+        t = ManyTuning()
+        _check(lib().sj_many_autotune(self.h, 1 if tune_form else 0, reps, C.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = ManyTuning()
+        _check(lib().sj_many_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, form, schedule, blocks_per_cu=8, queue_chunk=1):
+        if isinstance(form, str):
+            form = FORMS[form]
+        t = ManyTuning(form, schedule, blocks_per_cu, queue_chunk)
+        _check(lib().sj_many_set_tuning(self.h, C.byref(t)))
+
+    def times(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().sj_many_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(ms_plan=a.value, ms_grid=b.value, ms_run=c.value)
+
+    def stats(self, i=-1):
+        s = Stats()
+        _check(lib().sj_many_stats(self.h, i, C.byref(s)))
+        return s.as_dict()
+
+    def device_table(self, i):
+        o, d, r, tot = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint64()
+        _check(lib().sj_many_instance(self.h, i, C.byref(o), C.byref(d), C.byref(r), C.byref(tot)))
+        return o.value, d.value, r.value, tot.value
+
+    def csr(self, i):
+        """Host CSR of instance i in original point-ID order."""
+        n = self.shapes[i][0]
+        o, d, r, tot = self.device_table(i)
+        off = download(o, n + 1, np.uint64).astype(np.int64)
+        ids = download(d, tot, np.int32) if tot else np.zeros(0, dtype=np.int32)
+        rows = download(r, n, np.uint32).astype(np.int64)
+        cnt = np.diff(off)
+        out_cnt = np.zeros(n, dtype=np.int64)
+        out_cnt[rows] = cnt
+        out_off = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(out_cnt, out=out_off[1:])
+        out = np.empty(tot, dtype=np.int32)
+        # row r (cell order) of point rows[r] moves to out_off[rows[r]]
+        src = np.repeat(off[:-1], cnt)
+        dst = np.repeat(out_off[rows], cnt)
+        within = np.arange(tot, dtype=np.int64) - np.repeat(off[:-1], cnt)
+        out[dst + within] = ids[src + within]
+        return out_off.astype(np.uint64), out
+
