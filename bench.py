@@ -1,14 +1,26 @@
-"""Self-join benchmark: result pairs/s and join time at 2M points (2-D,
-Exp(40) clipped to [0,1], seed 2, eps for ~64 neighbours per point),
-workload sort + work queue + B-Unicomp, FP32+FP64-recheck vs plain FP64.
+"""Self-join benchmark: result pairs/s and join time, FP32+FP64-recheck vs
+plain FP64, with the roofline of the dominant kernel and a CPU baseline.
 
-One step = grid construction + count pass + offsets + fill pass over the
-2,000,000 points already resident in HBM, producing the exact neighbour
-table (CSR, original point IDs) in HBM.
+Workloads (--workload, default exp2d):
+  exp2d       2,000,000 points, 2-D Exp(40) clipped to [0,1], seed 2, eps for
+              ~64 neighbours/pt; workload-sorted work queue, B-Unicomp.
+  uniform2d   100,000 points, U[0,100]^2, seed 1, eps 0.5; Unicomp.
+  uniform6d   2,000,000 points, U[0,1]^6, seed 3, eps for ~32 neighbours/pt;
+              linear-ID Unicomp, result batches forced to >= 4 rounds.
+  clusters64d 500,000 points, 64-D, 100 Gaussian clusters, seed 4, 6 indexed
+              dims, eps for ~50 neighbours/pt; FP32 distances on the matrix
+              cores (roofline: FP32 MFMA).
+  many64      64 independent joins (N log-uniform 1e4..5e5, d in
+              {2,3,4,6,8}, uniform or Exp(40) data, 8..128 neighbours/pt,
+              seed 5) in one graph-recorded persistent launch per pass,
+              parameters picked by the autotuner.
 
-Prints one JSON line (rank 0).  The roofline object covers the dominant
-kernel, timed with HIP events on the stream it runs on; cpu_baseline is the
-CPU oracle (grid build + LID join, OpenMP) on the host cores.
+One step = grid construction + count pass + offsets + fill pass over points
+already resident in HBM, producing the exact neighbour table in HBM (for
+many64: one replay of the recorded graph over the planned grids).  Prints
+one JSON line.  roofline covers the dominant kernel, timed with HIP events
+on the stream it runs on; cpu_baseline is the CPU oracle (grid join,
+OpenMP) on the host cores.
 """
 import argparse
 import json
@@ -20,7 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "self-join-hpbdc2019_amd"))
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "exp2d"
+MFMA_FP32_PEAK_TFLOPS = 157.3
+METRIC = "result pairs/sec & join time at 2M pts, FP32+recheck vs FP64, % HBM roofline"
+
+SINGLE = {
+    "exp2d": dict(form="bunicomp", schedule="workqueue", rounds=0,
+                  desc="2,000,000 points x 2-D, Exp(40) clipped [0,1], seed 2, ~64 neighbours/pt"),
+    "uniform2d": dict(form="unicomp", schedule="strided", rounds=0,
+                      desc="100,000 points x 2-D, U[0,100]^2, seed 1"),
+    "uniform6d": dict(form="lid", schedule="strided", rounds=4,
+                      desc="2,000,000 points x 6-D, U[0,1]^6, seed 3, ~32 neighbours/pt, >= 4 result rounds"),
+    "clusters64d": dict(form="lid", schedule="strided", rounds=0,
+                        desc="500,000 points x 64-D, 100 Gaussian clusters (sigma 0.5), seed 4, "
+                             "6 indexed dims, ~50 neighbours/pt"),
+}
 
 
 def parse():
@@ -28,19 +53,34 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default=WORKLOAD)
-    ap.add_argument("--form", default="bunicomp")
-    ap.add_argument("--schedule", default="workqueue")
+    ap.add_argument("--workload", default="exp2d", choices=list(SINGLE) + ["many64"])
+    ap.add_argument("--form", default=None)
+    ap.add_argument("--schedule", default=None)
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="timed steps of the plain FP64 form (default: --steps; 1 for clusters64d)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--probe", action="store_true",
+                    help="counter probe: a 512 MiB device copy, then one FP32+recheck and one FP64 join "
+                         "of the workload (run under rocprofv3 --pmc)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two --probe counter runs (FETCH_SIZE, WRITE_SIZE) into "
+                         "profiles/traffic_<workload>_<kernel>.json")
     return ap.parse_args()
 
 
-def run_gpu(ctx, x, k, eps, prec, form, sched, steps, warmup, torch):
-    import sjamd
+def cores_used():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        cores = min(cores, int(env))
+    return cores
 
-    n, d = x.shape
-    kw = dict(form=form, precision=prec, schedule=sched)
+
+def run_gpu(ctx, x, k, eps, kw, steps, warmup, torch):
     stats = []
     for _ in range(warmup):
         r = ctx.self_join(x, k, eps, **kw)
@@ -54,44 +94,73 @@ def run_gpu(ctx, x, k, eps, prec, form, sched, steps, warmup, torch):
         total = r.total
         r.free()
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+    dt = (time.perf_counter() - t0) / max(steps, 1)
     avg = {key: sum(s[key] for s in stats) / len(stats) for key in stats[0]}
     return dt, total, avg
 
 
-def checksum(ctx, x, k, eps, prec, form, sched):
-    """Order-free fingerprint of the neighbour table: |S|, sum of
-    (row ID + 1) * neighbour ID and sum of squared IDs, on the device."""
-    import torch
+def fingerprint(off, ids, rows, torch):
+    """Order-free fingerprint of a neighbour table on the device: |S|, sum
+    of (row ID + 1) * neighbour ID and sum of squared IDs."""
+    rr = torch.repeat_interleave(rows.to(torch.int64), (off[1:] - off[:-1]).to(torch.int64))
+    v = ids.to(torch.int64)
+    return (int(off[-1].item() - off[0].item()), int((v * (rr + 1)).sum().item()), int((v * v).sum().item()))
 
+
+def checksum(ctx, x, k, eps, kw, torch):
     import sjamd
 
-    r = ctx.self_join(x, k, eps, form=form, precision=prec, schedule=sched)
-    off_p, ids_p, rid_p = r.device_table()
-    n = x.shape[0]
-    off = torch.from_numpy(sjamd.download(off_p, n + 1, "int64")).to(x.device)
-    ids = torch.from_numpy(sjamd.download(ids_p, r.total, "int32")).to(x.device)
-    rid = torch.from_numpy(sjamd.download(rid_p, n, "int32")).to(x.device)
-    rows = torch.repeat_interleave(rid.to(torch.int64), off[1:] - off[:-1])
-    v = ids.to(torch.int64)
-    fp = (int(off[-1].item()), int((v * (rows + 1)).sum().item()), int((v * v).sum().item()))
+    r = ctx.self_join(x, k, eps, **kw)
+    if r.stats()["num_batches"] > 1:
+        off, ids = r.csr()  # batches live on the host
+        rows = torch.arange(x.shape[0], device=x.device)
+        fp = fingerprint(torch.from_numpy(off.astype("int64")).to(x.device),
+                         torch.from_numpy(ids).to(x.device), rows, torch)
+    else:
+        o, i, rid = r.device_table()
+        n = x.shape[0]
+        fp = fingerprint(sjamd.device_copy(o, n + 1, torch.int64), sjamd.device_copy(i, r.total, torch.int32),
+                         sjamd.device_copy(rid, n, torch.int32), torch)
     r.free()
     return fp
 
 
-def cpu_baseline(points, k, eps, reps):
+def canonical_pairs_C(g):
+    """Unordered candidate pairs of the canonical grid (cell width eps):
+    sum over cells n_c(n_c+1)/2 plus sum over adjacent unordered cell pairs
+    n_a n_b, from the device grid's occupancy."""
+    import itertools
+
+    import numpy as np
+
+    a = g.arrays()
+    lin = a["cellLinear"].astype(np.int64)
+    size = (a["indexMax"] - a["indexMin"] + 1).astype(np.int64)
+    coord = a["cellCoord"].astype(np.int64)
+    _, _, nc, _ = g.dims()
+    k = coord.shape[1]
+    mult = np.cumprod(np.concatenate([[1], nc[:-1].astype(np.int64)]))
+    pos = {int(l): i for i, l in enumerate(lin)}
+    C = int((size * (size + 1) // 2).sum())
+    offs = [o for o in itertools.product([-1, 0, 1], repeat=k) if o > tuple([0] * k)]
+    for o in offs:
+        o = np.array(o, dtype=np.int64)
+        c2 = coord + o
+        ok = np.all((c2 >= 0) & (c2 < nc.astype(np.int64)), axis=1)
+        l2 = (c2 * mult).sum(axis=1)
+        for i in np.nonzero(ok)[0]:
+            j = pos.get(int(l2[i]))
+            if j is not None:
+                C += int(size[i] * size[j])
+    return C
+
+
+def cpu_baseline_join(points, k, eps, reps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    env = os.environ.get("OMP_NUM_THREADS")
-    if env and env.isdigit():
-        cores = min(cores, int(env))
-    best = None
-    total = 0
+    cores = cores_used()
+    best, total = None, 0
     for _ in range(reps):
         t0 = time.perf_counter()
         g = O.OracleGrid(points, k, eps)
@@ -101,59 +170,87 @@ def cpu_baseline(points, k, eps, reps):
         del g, off, ids
         best = dt if best is None else min(best, dt)
     return {"value": total / best, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"full workload ({points.shape[0]} points, grid build + LID join), "
-                      f"best of {reps}", "seconds": best}
+            "sample": f"full workload ({points.shape[0]} points, grid build + LID join), best of {reps}",
+            "seconds": best}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the join shards by query cells; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
-    import torch
+def cpu_baseline_sampled(points, k, eps, stride):
+    """Grid join of every stride-th query point against the whole grid (full
+    stencil): pairs found per second of that work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
 
+    cores = cores_used()
+    g = O.OracleGrid(points, k, eps)
+    t0 = time.perf_counter()
+    cnt = int(g.sample_count(stride))
+    dt = time.perf_counter() - t0
+    return {"value": cnt / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"queries = every {stride}th of {points.shape[0]} points against the whole grid "
+                      f"(full stencil, {cnt} pairs, grid build excluded)", "seconds": dt}
+
+
+def bench_single(a, torch):
     import sjamd
     from sjamd import datasets as D
 
+    spec = SINGLE[a.workload]
     w = D.WORKLOADS[a.workload]
     pts = D.make_points(a.workload)
     eps = D.workload_eps(a.workload)
     k = w["k"]
     n, d = pts.shape
-    form = sjamd.FORMS[a.form]
-    sched = sjamd.WORKQUEUE if a.schedule == "workqueue" else sjamd.STRIDED
+    form = sjamd.FORMS[a.form or spec["form"]]
+    sched_name = a.schedule or spec["schedule"]
+    sched = sjamd.WORKQUEUE if sched_name == "workqueue" else sjamd.STRIDED
+    fp64_steps = a.fp64_steps if a.fp64_steps is not None else (1 if a.workload == "clusters64d" else a.steps)
 
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         ctx = sjamd.Context(0, stream=stream)
         x = ctx.to_device(pts)
         torch.cuda.synchronize()
-        fp64 = run_gpu(ctx, x, k, eps, sjamd.FP64, form, sched, a.steps, a.warmup, torch)
-        fp32 = run_gpu(ctx, x, k, eps, sjamd.FP32_RECHECK, form, sched, a.steps, a.warmup, torch)
-        fp_a = checksum(ctx, x, k, eps, sjamd.FP64, form, sched)
-        fp_b = checksum(ctx, x, k, eps, sjamd.FP32_RECHECK, form, sched)
+        kw = dict(form=form, schedule=sched)
+        if spec["rounds"]:
+            r = ctx.self_join(x, k, eps, **kw)
+            kw["capacity"] = r.total // spec["rounds"] + 1
+            r.free()
+        fp32 = run_gpu(ctx, x, k, eps, dict(kw, precision=sjamd.FP32_RECHECK), a.steps, a.warmup, torch)
+        fp64 = run_gpu(ctx, x, k, eps, dict(kw, precision=sjamd.FP64), fp64_steps,
+                       min(a.warmup, fp64_steps), torch)
+        fp_a = checksum(ctx, x, k, eps, dict(kw, precision=sjamd.FP64), torch)
+        fp_b = checksum(ctx, x, k, eps, dict(kw, precision=sjamd.FP32_RECHECK), torch)
+        C = None
+        if a.workload == "clusters64d":
+            g = ctx.grid_build(x, k, eps)
+            C = canonical_pairs_C(g)
+            g.free()
     exact = fp_a == fp_b
 
     dt, total, st = fp32
     ms = dt * 1e3
-    # dominant kernel: the longer of the two traversal passes
-    b_alg_fill = n * d * 8 + total * 4 + (n + 1) * 8
-    b_alg_count = n * d * 8 + n * 4 * 2
     if st["ms_fill"] >= st["ms_count"]:
-        kname, kms, kbytes = "fill", st["ms_fill"], b_alg_fill
+        kname, kms = "fill", st["ms_fill"]
     else:
-        kname, kms, kbytes = "count", st["ms_count"], b_alg_count
-    achieved = kbytes / (kms * 1e-3) / 1e9
+        kname, kms = "count", st["ms_count"]
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_{kname}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-
+    if a.workload == "clusters64d":
+        flops = 2.0 * d * C
+        achieved = flops / (kms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / MFMA_FP32_PEAK_TFLOPS, "traffic": traffic,
+                "alg_flops": flops, "canonical_pairs_C": C, "kernel_ms": kms}
+    else:
+        kbytes = n * d * 8 + total * 4 + (n + 1) * 8 if kname == "fill" else n * d * 8 + n * 4 * 2
+        achieved = kbytes / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": kbytes, "kernel_ms": kms}
     out = {
-        "metric": "result pairs/sec & join time at 2M pts, FP32+recheck vs FP64, % HBM roofline",
+        "metric": METRIC,
         "value": total / dt,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -165,24 +262,201 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.workload}: {n} points x {d}-D, Exp(40) clipped [0,1], seed 2, "
-                               f"eps={eps.hex()} (~64 neighbours/pt)",
-                   "form": a.form, "schedule": a.schedule, "precision": "fp32+fp64-recheck",
-                   "pairs": total, "exact_vs_fp64": exact},
-        "fp64": {"ms_per_step": fp64[0] * 1e3, "pairs_per_s": fp64[1] / fp64[0],
+        "config": {"workload": f"{a.workload}: {spec['desc']}, eps={eps.hex()}",
+                   "form": a.form or spec["form"], "schedule": sched_name,
+                   "precision": "fp32+fp64-recheck", "pairs": total, "exact_vs_fp64": exact,
+                   "result_rounds": int(st["num_batches"])},
+        "fp64": {"ms_per_step": fp64[0] * 1e3, "pairs_per_s": fp64[1] / fp64[0], "steps": fp64_steps,
                  "ms_count": fp64[2]["ms_count"], "ms_fill": fp64[2]["ms_fill"]},
         "fp32_recheck": {"ms_per_step": ms, "ms_grid": st["ms_grid"], "ms_workload": st["ms_workload"],
                          "ms_count": st["ms_count"], "ms_scan": st["ms_scan"], "ms_fill": st["ms_fill"],
                          "band_pairs": st["fp32_band_pairs"],
                          "misclassified_before_recheck": st["fp32_misclassified"],
                          "candidates": st["candidates"]},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes": kbytes, "kernel_ms": kms},
+        "fp32_speedup_vs_fp64": fp64[0] / dt,
+        "roofline": roof,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pts, k, eps, a.cpu_reps)
+        if a.workload == "clusters64d":
+            out["cpu_baseline"] = cpu_baseline_sampled(pts, k, eps, 1000)
+        else:
+            out["cpu_baseline"] = cpu_baseline_join(pts, k, eps, a.cpu_reps)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def bench_many(a, torch):
+    import sjamd
+    from sjamd import datasets as D
+
+    insts = D.multi_join_instances()
+    batch = [(D.instance_points(i), i["d"], D.multi_eps(j)) for j, i in enumerate(insts)]
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx = sjamd.Context(0, stream=stream)
+        res = {}
+        for name, prec in [("fp32", sjamd.FP32_RECHECK), ("fp64", sjamd.FP64)]:
+            m = ctx.many(batch, form="bunicomp", precision=prec, schedule=sjamd.WORKQUEUE)
+            tuned = m.autotune(tune_form=True, reps=2)
+            for _ in range(a.warmup):
+                m.run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dev = 0.0
+            for _ in range(a.steps):
+                m.run()
+                dev += m.times()["ms_run"]
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / max(a.steps, 1)
+            st = m.stats()
+            fps = []
+            for i in range(len(batch)):
+                o, ids, rid, tot = m.device_table(i)
+                n = batch[i][0].shape[0]
+                fps.append(fingerprint(sjamd.device_copy(o, n + 1, torch.int64),
+                                       sjamd.device_copy(ids, tot, torch.int32),
+                                       sjamd.device_copy(rid, n, torch.int32), torch))
+            res[name] = dict(dt=dt, dev_ms=dev / max(a.steps, 1), total=st["total_pairs"], stats=st,
+                             times=m.times(), tuning=tuned, fps=fps)
+            m.free()
+    f32 = res["fp32"]
+    total = f32["total"]
+    exact = res["fp32"]["fps"] == res["fp64"]["fps"]
+    kbytes = sum(p.shape[0] * p.shape[1] * 8 + (p.shape[0] + 1) * 8 for p, _, _ in batch) + total * 4
+    kms = f32["dev_ms"]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    names = {v: k for k, v in sjamd.FORMS.items()}
+    tune = dict(f32["tuning"])
+    tune["form"] = names[tune["form"]]
+    tune["schedule"] = "workqueue" if tune["schedule"] == sjamd.WORKQUEUE else "strided"
+    out = {
+        "metric": METRIC,
+        "value": total / f32["dt"],
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": f32["dt"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "many64: 64 joins, N log-uniform 1e4..5e5, d in {2,3,4,6,8}, uniform or "
+                               "Exp(40) data, 8..128 neighbours/pt, seed 5; one step = one graph replay "
+                               "(count + offsets + fill of all 64 joins over the planned grids)",
+                   "instances": len(batch), "points": int(sum(p.shape[0] for p, _, _ in batch)),
+                   "pairs": total, "exact_vs_fp64": exact, "autotuned": tune,
+                   "precision": "fp32+fp64-recheck"},
+        "plan_ms": f32["times"]["ms_plan"], "plan_grid_ms": f32["times"]["ms_grid"],
+        "fp64": {"ms_per_step": res["fp64"]["dt"] * 1e3, "pairs_per_s": res["fp64"]["total"] / res["fp64"]["dt"],
+                 "autotuned": res["fp64"]["tuning"]},
+        "fp32_recheck": {"ms_per_step": f32["dt"] * 1e3, "graph_device_ms": kms,
+                         "band_pairs": f32["stats"]["fp32_band_pairs"],
+                         "misclassified_before_recheck": f32["stats"]["fp32_misclassified"],
+                         "candidates": f32["stats"]["candidates"]},
+        "fp32_speedup_vs_fp64": res["fp64"]["dt"] / f32["dt"],
+        "roofline": {"bound": "hbm", "kernel": "graph replay (count + offsets + fill, all instances)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes": kbytes, "kernel_ms": kms},
+    }
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        cores = cores_used()
+        t0 = time.perf_counter()
+        tot = 0
+        for p, k, e in batch:
+            off, _ = O.OracleGrid(p, k, e).join(O.LID, nthreads=cores, sort_rows=False)
+            tot += int(off[-1])
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": tot / sec, "unit": "pairs/s", "cores": cores, "kind": "port",
+                               "sample": "all 64 joins (grid build + LID join each), one pass", "seconds": sec}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+COPY_BYTES = 512 * 2**20
+
+
+def probe(a, torch):
+    import sjamd
+    from sjamd import datasets as D
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)  # the calibration dispatch: COPY_BYTES read, COPY_BYTES written
+    torch.cuda.synchronize()
+    del src, dst
+    spec = SINGLE[a.workload]
+    pts = D.make_points(a.workload)
+    eps = D.workload_eps(a.workload)
+    k = D.WORKLOADS[a.workload]["k"]
+    ctx = sjamd.Context(0)
+    x = ctx.to_device(pts)
+    form = sjamd.FORMS[a.form or spec["form"]]
+    sched = sjamd.WORKQUEUE if (a.schedule or spec["schedule"]) == "workqueue" else sjamd.STRIDED
+    for prec in (sjamd.FP32_RECHECK, sjamd.FP64):
+        r = ctx.self_join(x, k, eps, form=form, schedule=sched, precision=prec)
+        print(json.dumps({"precision": prec, "pairs": r.total, "ms_fill": r.stats()["ms_fill"]}), flush=True)
+        r.free()
+
+
+def traffic_from(a):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the traversal kernels,
+    scaled by what the same counter reads for the calibration copy."""
+    import csv
+    import glob
+
+    def read(d, counter):
+        rows = []
+        for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if r["Counter_Name"] == counter:
+                        rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        rows.sort()
+        return rows
+
+    out = {}
+    for counter, d in (("FETCH_SIZE", a.traffic_from[0]), ("WRITE_SIZE", a.traffic_from[1])):
+        rows = read(d, counter)
+        first_sj = next(i for i, r in enumerate(rows) if "sj::" in r[1])
+        copy = max(rows[:first_sj], key=lambda r: r[2])
+        factor = (COPY_BYTES / 1024.0) / copy[2]
+        joins = [r for r in rows if "k_join" in r[1]]
+        kern = joins[1]  # probe order: FP32+recheck count, fill, then FP64 count, fill
+        out[counter] = {"copy_kernel": copy[1], "copy_kib": copy[2], "scale": factor,
+                        "kernel": kern[1], "kib": kern[2], "bytes_scaled": kern[2] * 1024.0 * factor}
+    res = {"workload": a.workload, "kernel": out["FETCH_SIZE"]["kernel"],
+           "fetch_bytes": out["FETCH_SIZE"]["bytes_scaled"], "write_bytes": out["WRITE_SIZE"]["bytes_scaled"],
+           "hbm_bytes_per_launch": out["FETCH_SIZE"]["bytes_scaled"] + out["WRITE_SIZE"]["bytes_scaled"],
+           "calibration": {c: {k: v for k, v in out[c].items() if k != "kernel"} for c in out},
+           "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- python3 bench.py --probe "
+                      f"--workload {a.workload}; python3 bench.py --workload {a.workload} --traffic-from F W"}
+    path = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_fill.json")
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
+def main():
+    a = parse()
+    if a.traffic_from:
+        traffic_from(a)
+        return
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the join would shard by query cells over a replicated grid)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if a.probe:
+        probe(a, torch)
+        return
+    out = bench_many(a, torch) if a.workload == "many64" else bench_single(a, torch)
     print(json.dumps(out))
 
 

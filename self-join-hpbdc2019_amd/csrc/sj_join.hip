@@ -1212,6 +1212,8 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
 static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, const MfmaConsts *mc) {
     hipStream_t s = g->ctx->stream;
     if (mc && prec == SJ_PREC_FP32_RECHECK) return launch_join_mfma(a, *mc, pass, g->ctx->cu_count, s);
+    if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
+        return launch_join_wide(a, pass, g->ctx->cu_count, s);
     JoinKernel kfn = pick_kernel(g->k, g->d, pass, prec);
     if (!kfn) {
         set_error("no traversal kernel for k=%d", g->k);

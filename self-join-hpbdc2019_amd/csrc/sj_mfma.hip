@@ -30,54 +30,6 @@ namespace sj {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// Runtime-k form of the claim rules of claims<K>.
-__device__ __forceinline__ bool claims_rt(int form, int k, const uint32_t (&c)[DEV_MAXK],
-                                          const int (&o)[DEV_MAXK], uint64_t linA, uint64_t linB) {
-    if (form == SJ_FORM_FULL) return true;
-    if (form == SJ_FORM_LID) return linB > linA;
-    if (form == SJ_FORM_UNICOMP) {
-        bool found = false, odd = false;
-#pragma unroll
-        for (int j = DEV_MAXK - 1; j >= 0; j--) {
-            if (j < k && !found && o[j] != 0) {
-                odd = (c[j] & 1u) != 0;
-                found = true;
-            }
-        }
-        return odd;
-    }
-    uint32_t sc = 0;
-#pragma unroll
-    for (int j = 0; j < DEV_MAXK; j++)
-        if (j < k) sc += c[j];
-    const bool odd = sc & 1u;
-    if (k == 2) {
-        if (odd) return o[1] >= 0 && o[0] != 0;
-        return o[1] > 0 || (o[1] < 0 && o[0] == 0);
-    }
-    if (k == 3) {
-        if (odd)
-            return (o[0] != 0 && o[1] >= 0 && o[2] >= 0) || (o[0] == 0 && o[1] >= 0 && o[2] > 0) ||
-                   (o[1] > 0 && o[2] < 0);
-        return (o[1] > 0 && o[2] >= 0) || (o[0] == 0 && o[1] < 0 && o[2] >= 0) ||
-               (o[0] != 0 && o[1] > 0 && o[2] < 0) || (o[1] == 0 && o[2] > 0);
-    }
-    int so = 0;
-    bool pos = false, found = false;
-#pragma unroll
-    for (int j = DEV_MAXK - 1; j >= 0; j--) {
-        if (j < k) {
-            so += o[j];
-            if (!found && o[j] != 0) {
-                pos = o[j] > 0;
-                found = true;
-            }
-        }
-    }
-    const bool even = (so & 1) == 0;
-    return pos != (even && odd);
-}
-
 __device__ __forceinline__ uint32_t range_bits32(uint32_t start, uint32_t A, uint32_t B) {
     return (uint32_t)range_bits(start, A, B);  // start..start+31 only: low word
 }

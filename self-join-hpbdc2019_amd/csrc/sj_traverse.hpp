@@ -172,6 +172,54 @@ __device__ __forceinline__ bool claims(int form, const uint32_t (&c)[K], const i
     }
 }
 
+// Runtime-k form of the claim rules of claims<K>.
+__device__ __forceinline__ bool claims_rt(int form, int k, const uint32_t (&c)[DEV_MAXK],
+                                          const int (&o)[DEV_MAXK], uint64_t linA, uint64_t linB) {
+    if (form == SJ_FORM_FULL) return true;
+    if (form == SJ_FORM_LID) return linB > linA;
+    if (form == SJ_FORM_UNICOMP) {
+        bool found = false, odd = false;
+#pragma unroll
+        for (int j = DEV_MAXK - 1; j >= 0; j--) {
+            if (j < k && !found && o[j] != 0) {
+                odd = (c[j] & 1u) != 0;
+                found = true;
+            }
+        }
+        return odd;
+    }
+    uint32_t sc = 0;
+#pragma unroll
+    for (int j = 0; j < DEV_MAXK; j++)
+        if (j < k) sc += c[j];
+    const bool odd = sc & 1u;
+    if (k == 2) {
+        if (odd) return o[1] >= 0 && o[0] != 0;
+        return o[1] > 0 || (o[1] < 0 && o[0] == 0);
+    }
+    if (k == 3) {
+        if (odd)
+            return (o[0] != 0 && o[1] >= 0 && o[2] >= 0) || (o[0] == 0 && o[1] >= 0 && o[2] > 0) ||
+                   (o[1] > 0 && o[2] < 0);
+        return (o[1] > 0 && o[2] >= 0) || (o[0] == 0 && o[1] < 0 && o[2] >= 0) ||
+               (o[0] != 0 && o[1] > 0 && o[2] < 0) || (o[1] == 0 && o[2] > 0);
+    }
+    int so = 0;
+    bool pos = false, found = false;
+#pragma unroll
+    for (int j = DEV_MAXK - 1; j >= 0; j--) {
+        if (j < k) {
+            so += o[j];
+            if (!found && o[j] != 0) {
+                pos = o[j] > 0;
+                found = true;
+            }
+        }
+    }
+    const bool even = (so & 1) == 0;
+    return pos != (even && odd);
+}
+
 // FP64 canonical predicate: dimension order, explicit fma.
 template <int D>
 __device__ __forceinline__ double d2_fp64(const double *p, const double *q, int d) {
@@ -223,5 +271,9 @@ sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, con
 // Work-queue keys of tiles [0, bound) of one join (descending 8-bit
 // log-scale bucket of the staged candidates; val[t] = t).
 sj_status launch_tile_cost(const JoinArgs &a, int k, uint32_t bound, uint32_t *key, uint32_t *val, hipStream_t s);
+
+// Plain FP64 traversal from LDS-staged candidates for d != k or d > 8, d <= 64.
+bool wide_supported(int d, int k);
+sj_status launch_join_wide(const JoinArgs &a, int pass, int cu_count, hipStream_t s);
 
 }  // namespace sj

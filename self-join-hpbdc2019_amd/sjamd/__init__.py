@@ -210,6 +210,21 @@ def download(ptr, count, dtype):
     return out
 
 
+def device_copy(ptr, count, torch_dtype, device=0):
+    """Copy `count` elements of a device array into a new torch tensor on
+    the same device (device to device)."""
+    import torch
+
+    out = torch.empty(count, dtype=torch_dtype, device=f"cuda:{device}")
+    if count:
+        torch.cuda.synchronize()
+        rc = _hip().hipMemcpy(C.c_void_p(out.data_ptr()), C.c_void_p(ptr),
+                              count * out.element_size(), 3)
+        if rc != 0:
+            raise SJError(3, f"hipMemcpy device->device failed ({rc})")
+    return out
+
+
 def make_params(form=LID, precision=FP64, schedule=STRIDED, sort_rows=False, capacity=0,
                 queue_chunk=0, per_point=False):
     """per_point=True disables the LDS tile walk (every lane walks its

@@ -166,6 +166,38 @@ def multi_join_instances(count=64, seed=5):
     return out
 
 
+# eps of the 64 instances of multi_join_instances(), calibrated by
+# calibrate_eps() for each instance's mean neighbour count, exact floats.
+MULTI_EPS = [
+    "0x1.c7ac2079d3230p-3", "0x1.441015ec6cbfep-8", "0x1.cbabd26821592p-3",
+    "0x1.4a5c01e31fcd8p-8", "0x1.5ec56f999a254p-4", "0x1.0ddc06dfddc34p-1",
+    "0x1.d9927b589bdf0p-7", "0x1.6fb12e2c63961p-2", "0x1.ea7938c6a4d76p-10",
+    "0x1.41a61a73223e2p-4", "0x1.da5ebe7ccde80p-3", "0x1.099bd2d4c3714p-2",
+    "0x1.e58ac1f9a7681p-4", "0x1.e8b02f6d36852p-7", "0x1.bd9ab3ec121f9p-7",
+    "0x1.26161c3722ae6p-6", "0x1.42623ee446ab0p-4", "0x1.7295695a76d6cp-6",
+    "0x1.23b34847eb63fp-12", "0x1.4f3d7076d62b8p-8", "0x1.9189f3776c8d4p-9",
+    "0x1.0518bd6b5c803p-11", "0x1.71e197df372c0p-6", "0x1.dfa87845775a8p-10",
+    "0x1.f939c8fc37dbep-10", "0x1.a33ae88560d14p-8", "0x1.1e9a9086bda64p-8",
+    "0x1.01b86517b7474p-7", "0x1.7bbabb2ad7b9fp-8", "0x1.3584a739f802ap-2",
+    "0x1.3a38250b09d81p-3", "0x1.b8600d695bbb6p-3", "0x1.9b7d67e0c9272p-2",
+    "0x1.ad9998c8c44a8p-5", "0x1.15b98acf43ecdp-9", "0x1.d6f17a8da50e6p-3",
+    "0x1.af04fb65d5e70p-12", "0x1.f6dd8b53db663p-5", "0x1.3f8f3fc1065d0p-4",
+    "0x1.f9c66e33b2019p-12", "0x1.08100abb0161cp-3", "0x1.1b575db728118p-7",
+    "0x1.fd0bcec26322bp-8", "0x1.b6f21e606a7c8p-7", "0x1.5028496e9d450p-2",
+    "0x1.a37439b500205p-7", "0x1.411c723dc04afp-10", "0x1.81ff2fbc23542p-7",
+    "0x1.1dfce4adb1d86p-6", "0x1.9dc0670ec4225p-5", "0x1.89f8258ad3ed7p-2",
+    "0x1.0bf33a30500b5p-2", "0x1.a945c7df6b6c8p-7", "0x1.09934cab221f1p-9",
+    "0x1.2fab88d62ece4p-2", "0x1.53d565431e040p-8", "0x1.94005430b77bcp-2",
+    "0x1.1e73d67c6c04bp-2", "0x1.6167a222af96ap-8", "0x1.ceb247a594c42p-11",
+    "0x1.7434d168536c1p-7", "0x1.6aeb644ff0229p-8", "0x1.a6051f73f88c4p-3",
+    "0x1.d870e00ac308ap-6",
+]
+
+
+def multi_eps(i):
+    return float.fromhex(MULTI_EPS[i])
+
+
 def instance_points(inst):
     if inst["kind"] == "uniform":
         return uniform(inst["n"], inst["d"], inst["seed"])
