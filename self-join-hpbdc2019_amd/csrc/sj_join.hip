@@ -29,7 +29,9 @@
 #include <cstring>
 
 #include "sj_common.hpp"
+#include "sj_mfma.hpp"
 #include "sj_traverse.hpp"
+#include "sj_wide.hpp"
 
 namespace sj {
 
@@ -622,8 +624,10 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
 // ------------------------------------------------------------ many joins
 // One persistent launch over the tiles of many independent joins: work
 // item w is tile workTile[w] of instance workInst[w], whose arguments are
-// args[instance].  Each tile runs the traversal specialised for its
-// instance's (k, d, precision); statistics go to the instance's counters.
+// args[instance].  FP32 instances run the FP32 LDS tile walk when
+// d == k <= 8 and the matrix-core walk otherwise; FP64 ones the LDS tile
+// walk when d == k <= 8 and the wide walk otherwise.  Statistics go to the
+// instance's counters.
 template <int A, int B>
 struct cmax {
     static constexpr int v = A > B ? A : B;
@@ -635,27 +639,33 @@ struct LdsMax {
 };
 template <>
 struct LdsMax<0> {
-    static constexpr int v = 0;
+    static constexpr int v = cmax<(int)sizeof(WideLds), (int)sizeof(MfmaLds<64>)>::v;
 };
 constexpr int MANY_LDS = (LdsMax<DEV_MAXK>::v + 15) / 16 * 16;
 
-template <int K, int PASS>
+template <int K, int PASS, int PREC>
 __device__ __forceinline__ void many_tile(const JoinArgs &a, uint32_t tile, unsigned char *raw, LaneStats &st) {
-    const bool fp32 = a.pts32 != nullptr;
-    if (a.d == K && a.useTiles) {
-        if (fp32) process_tile<K, K, PASS, 1>(a, tile, *reinterpret_cast<TileLds<K, 1> *>(raw), st);
-        else process_tile<K, K, PASS, 0>(a, tile, *reinterpret_cast<TileLds<K, 0> *>(raw), st);
-    } else {
-        const uint64_t s = (uint64_t)a.tileStart[tile] + __lane_id();
-        if (s < a.tileStart[tile + 1]) {
-            if (fp32) process_point<K, 0, PASS, 1>(a, (uint32_t)s, st);
-            else process_point<K, 0, PASS, 0>(a, (uint32_t)s, st);
-        }
-    }
+    if (PREC == 1) process_tile<K, K, PASS, 1>(a, tile, *reinterpret_cast<TileLds<K, 1> *>(raw), st);
+    else if (a.d == K) process_tile<K, K, PASS, 0>(a, tile, *reinterpret_cast<TileLds<K, 0> *>(raw), st);
+    else process_tile_wide<PASS>(a, tile, *reinterpret_cast<WideLds *>(raw), st);
 }
 
-template <int PASS>
-__global__ __launch_bounds__(JB) void k_join_many(const JoinArgs *__restrict__ args,
+template <int DP, int PASS>
+__device__ __forceinline__ void many_mfma(const JoinArgs &a, uint32_t tile, unsigned char *raw, int &cleared,
+                                          LaneStats &st) {
+    // the pad columns [d, DP) of the staged rows must be zero: clear when
+    // (DP, d) changes or another walk used the LDS
+    MfmaLds<DP> &L = *reinterpret_cast<MfmaLds<DP> *>(raw);
+    const int key = DP * 128 + a.d;
+    if (cleared != key) {
+        mfma_lds_clear(L);
+        cleared = key;
+    }
+    process_tile_mfma<DP, PASS>(a, tile, L, st);
+}
+
+template <int PASS, int MF>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_many(const JoinArgs *__restrict__ args,
                                                   const uint32_t *__restrict__ workInst,
                                                   const uint32_t *__restrict__ workTile, uint32_t W,
                                                   uint32_t *qctr, uint32_t chunk, int useQueue) {
@@ -665,6 +675,7 @@ __global__ __launch_bounds__(JB) void k_join_many(const JoinArgs *__restrict__ a
     unsigned char *raw = lds[wave];
     LaneStats st;
     uint32_t cur = 0xFFFFFFFFu;
+    int cleared = 0;  // (DP, d) whose zero pad columns the LDS holds, 0: none
     auto run = [&](uint32_t w) __attribute__((always_inline)) {
         const uint32_t inst = (uint32_t)__builtin_amdgcn_readfirstlane((int)workInst[w]);
         const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)workTile[w]);
@@ -674,15 +685,26 @@ __global__ __launch_bounds__(JB) void k_join_many(const JoinArgs *__restrict__ a
             cur = inst;
         }
         const JoinArgs &a = args[inst];
+        if (MF && a.mDP > 0) {
+            switch (a.mDP) {
+            case 4: many_mfma<4, PASS>(a, tile, raw, cleared, st); break;
+            case 8: many_mfma<8, PASS>(a, tile, raw, cleared, st); break;
+            case 16: many_mfma<16, PASS>(a, tile, raw, cleared, st); break;
+            case 32: many_mfma<32, PASS>(a, tile, raw, cleared, st); break;
+            case 64: many_mfma<64, PASS>(a, tile, raw, cleared, st); break;
+            }
+            return;
+        }
+        cleared = 0;
         switch (a.k) {
-        case 1: many_tile<1, PASS>(a, tile, raw, st); break;
-        case 2: many_tile<2, PASS>(a, tile, raw, st); break;
-        case 3: many_tile<3, PASS>(a, tile, raw, st); break;
-        case 4: many_tile<4, PASS>(a, tile, raw, st); break;
-        case 5: many_tile<5, PASS>(a, tile, raw, st); break;
-        case 6: many_tile<6, PASS>(a, tile, raw, st); break;
-        case 7: many_tile<7, PASS>(a, tile, raw, st); break;
-        case 8: many_tile<8, PASS>(a, tile, raw, st); break;
+        case 1: many_tile<1, PASS, MF>(a, tile, raw, st); break;
+        case 2: many_tile<2, PASS, MF>(a, tile, raw, st); break;
+        case 3: many_tile<3, PASS, MF>(a, tile, raw, st); break;
+        case 4: many_tile<4, PASS, MF>(a, tile, raw, st); break;
+        case 5: many_tile<5, PASS, MF>(a, tile, raw, st); break;
+        case 6: many_tile<6, PASS, MF>(a, tile, raw, st); break;
+        case 7: many_tile<7, PASS, MF>(a, tile, raw, st); break;
+        case 8: many_tile<8, PASS, MF>(a, tile, raw, st); break;
         }
     };
     if (useQueue) {
@@ -1209,9 +1231,9 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
     return SJ_OK;
 }
 
-static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, const MfmaConsts *mc) {
+static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec) {
     hipStream_t s = g->ctx->stream;
-    if (mc && prec == SJ_PREC_FP32_RECHECK) return launch_join_mfma(a, *mc, pass, g->ctx->cu_count, s);
+    if (prec == SJ_PREC_FP32_RECHECK && a.mDP > 0) return launch_join_mfma(a, pass, g->ctx->cu_count, s);
     if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
         return launch_join_wide(a, pass, g->ctx->cu_count, s);
     JoinKernel kfn = pick_kernel(g->k, g->d, pass, prec);
@@ -1230,17 +1252,16 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
 }
 
 sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
-                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, unsigned blocks,
-                           hipStream_t s) {
+                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, int mfma,
+                           unsigned blocks, hipStream_t s) {
     if (W == 0) return SJ_OK;
     if (chunk == 0) chunk = 1;
     if (blocks == 0) blocks = 1;
-    if (pass == PASS_COUNT)
-        hipLaunchKernelGGL(k_join_many<PASS_COUNT>, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W,
-                           qctr, chunk, useQueue);
-    else
-        hipLaunchKernelGGL(k_join_many<PASS_FILL>, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W,
-                           qctr, chunk, useQueue);
+    using ManyKernel = void (*)(const JoinArgs *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, uint32_t,
+                                int);
+    const ManyKernel kfn = pass == PASS_COUNT ? (mfma ? k_join_many<PASS_COUNT, 1> : k_join_many<PASS_COUNT, 0>)
+                                              : (mfma ? k_join_many<PASS_FILL, 1> : k_join_many<PASS_FILL, 0>);
+    hipLaunchKernelGGL(kfn, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W, qctr, chunk, useQueue);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
@@ -1300,22 +1321,32 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
 
 sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base);
 
-// FP32 copy relative to the grid origin and the decision constants of the
-// precision form.  Falls back to FP64 (*prec) when the FP32 error bound
-// would not be useful.  One host sync (the coordinate bound).
-sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec, MfmaConsts *mc, const MfmaConsts **mcp) {
+// Decision constants of the precision form, from an FP32 copy relative to
+// the grid origin and its largest coordinate (one host sync).  With the LDS
+// tiles the matrix cores take d > 8 or d != k (d <= 64): they stage from
+// the FP32 copy when the coordinates lie within 1024 eps of the origin and
+// centre each tile in FP64 otherwise.  The FP32 VALU walk (d == k <= 8, or
+// no tiles) uses the global margin of fp32_margin and falls back to FP64
+// (*prec) when that margin would not be useful.
+bool use_mfma(const sj_grid_s *g, int useTiles) {
+    return useTiles && mfma_padded_dims(g->d) && (g->d > DEV_MAXK || g->d != g->k);
+}
+
+sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec) {
     sj_context ctx = g->ctx;
     Pool *pool = &ctx->pool;
     hipStream_t s = ctx->stream;
     const uint64_t n = g->n;
-    *mcp = nullptr;
+    a.mDP = 0;
+    a.mSrc32 = 0;
+    const bool mf = use_mfma(g, a.useTiles);
     DBuf<unsigned int> rbits;
     SJ_TRY(rbits.alloc(pool, 1));
     SJ_TRY(fill_u32(s, rbits.p, 1, 0u));
-    // d in the matrix-core range: rows padded for the MFMA operands
-    const int DP = a.useTiles ? mfma_padded_dims(g->d) : 0;
-    const int stride = DP ? DP : g->d;
-    if (!g->sortedPts32.p) SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)stride));
+    // rows padded to the matrix-core operand width for the matrix cores
+    const int stride = mf ? mfma_padded_dims(g->d) : g->d;
+    if (!g->sortedPts32.p || g->sortedPts32.n < n * (uint64_t)stride)
+        SJ_TRY(g->sortedPts32.alloc(pool, n * (uint64_t)stride));
     hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n)), dim3(JB), 0, s, g->sortedPts.p, n, g->d, g->k, stride,
                        g->d_gp.p, g->sortedPts32.p, rbits.p);
     SJ_LAUNCH_CHECK();
@@ -1327,15 +1358,17 @@ sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec, MfmaConsts *mc, con
     memcpy(&rf, &hb, 4);
     // the FP32 rounding of |v| may be below |v|: widen by one FP32 ulp
     const double R = (double)std::nextafter(rf, INFINITY);
+    if (mf) {
+        // FP32 copy for the matrix cores when the coordinates lie within
+        // 1024 eps of the origin; FP64 centring otherwise
+        const double Re = R * std::sqrt((double)g->d) * (1.0 + std::ldexp(1.0, -20));
+        const bool src32 = Re <= 1024.0 * std::sqrt(a.T);
+        mfma_consts(a, a.T, src32 ? Re : -1.0);
+        if (src32) a.pts32 = g->sortedPts32.p;
+        return SJ_OK;
+    }
     Fp32Margin m = fp32_margin(a.T, R, g->d);
-    bool mok = false;
-    if (DP) *mc = mfma_consts(a.T, R, g->d, DP, &mok);
-    if (DP && mok) {
-        a.pts32 = g->sortedPts32.p;
-        *mcp = mc;
-    } else if (DP) {
-        *prec = SJ_PREC_FP64;  // FP32 copy would not bound the error usefully
-    } else if (m.usable) {
+    if (m.usable) {
         a.pts32 = g->sortedPts32.p;
         a.lo32 = m.lo;
         a.hi32 = m.hi;
@@ -1379,9 +1412,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 
     // precision form: FP32 copy relative to the grid origin + margin
     int prec = P->precision;
-    MfmaConsts mc;
-    const MfmaConsts *mcp = nullptr;
-    if (prec == SJ_PREC_FP32_RECHECK) SJ_TRY(prepare_fp32(g, a, &prec, &mc, &mcp));
+    if (prec == SJ_PREC_FP32_RECHECK) SJ_TRY(prepare_fp32(g, a, &prec));
 
     // row-aligned tiles; work queue of tiles in descending workload
     DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr;
@@ -1410,7 +1441,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     a.mirCnt = mir.p;
     a.stat = stat.p;
     tc.start();
-    SJ_TRY(launch_join(g, a, PASS_COUNT, prec, mcp));
+    SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();
 
     // row offsets in cell order
@@ -1457,7 +1488,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         a.outBase = 0;
         if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
         tf.start();
-        SJ_TRY(launch_join(g, a, PASS_FILL, prec, mcp));
+        SJ_TRY(launch_join(g, a, PASS_FILL, prec));
         tf.stop();
         if (P->sort_rows) {
             tr.start();
@@ -1535,7 +1566,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             ab.wEnd = s1;
             ab.outBase = base;
             tf.start();
-            SJ_TRY(launch_join(g, ab, PASS_FILL, prec, mcp));
+            SJ_TRY(launch_join(g, ab, PASS_FILL, prec));
             tf.stop();
             if (P->sort_rows) SJ_TRY(sort_rows_device(ctx, r->d_offsets.p + s0, dbuf.p, s1 - s0, base));
             const uint64_t cnt = hOff[s1] - base;

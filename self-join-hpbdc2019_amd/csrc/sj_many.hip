@@ -100,6 +100,7 @@ struct sj_many_s {
     std::vector<uint64_t> hIdsBase;
     std::vector<unsigned long long> hStat;
     double ms_plan = 0.0, ms_grid = 0.0, ms_run = 0.0;
+    int mfma = 0;  // 1: every instance runs the matrix-core FP32 walk
 
     ~sj_many_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -132,7 +133,8 @@ sj_status enqueue_pipeline(sj_many_s *m, bool fill) {
     SJ_TRY(fill_u32(s, m->mir.p, N, 0u));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(m->stat.p), (uint64_t)cnt * 8, 0ull));
     SJ_TRY(fill_u32(s, m->qctr.p, 1, 0u));
-    SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_COUNT, blocks, s));
+    SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_COUNT, m->mfma, blocks,
+                           s));
     hipLaunchKernelGGL(k_many_totals, dim3(nb(N)), dim3(JB), 0, s, m->own.p, m->mir.p, N, m->tot.p);
     SJ_LAUNCH_CHECK();
     SJ_TRY(scan_exclusive_u32_to_u64_tmp(s, m->tot.p, N, m->G.p, m->scanTmp.p));
@@ -140,7 +142,8 @@ sj_status enqueue_pipeline(sj_many_s *m, bool fill) {
     hipLaunchKernelGGL(k_many_cursors, dim3(nb(N)), dim3(JB), 0, s, m->G.p, m->own.p, N, m->mirCur.p);
     SJ_LAUNCH_CHECK();
     SJ_TRY(fill_u32(s, m->qctr.p, 1, 0u));
-    SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_FILL, blocks, s));
+    SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_FILL, m->mfma, blocks,
+                           s));
     const uint64_t E = N + (uint64_t)cnt;
     hipLaunchKernelGGL(k_many_rebase, dim3(nb(E)), dim3(JB), 0, s, m->G.p, m->base.p, m->relBase.p, cnt, E,
                        m->rel.p, m->idsBase.p);
@@ -252,15 +255,18 @@ sj_status plan(sj_many_s *m, const sj_instance *in, int count) {
         I.base = N;
         N += x.n;
         JoinArgs a = base_args(I.g, P.form, eps_threshold(x.eps));
-        a.useTiles = x.d == x.k ? 1 : 0;  // LDS tiles for d == k, per-point walk otherwise
         int prec = P.precision;
-        if (prec == SJ_PREC_FP32_RECHECK) {
-            MfmaConsts mc;
-            const MfmaConsts *mcp = nullptr;
-            SJ_TRY(prepare_fp32(I.g, a, &prec, &mc, &mcp));
+        a.useTiles = 1;
+        if (prec == SJ_PREC_FP32_RECHECK) SJ_TRY(prepare_fp32(I.g, a, &prec));
+        if (P.precision == SJ_PREC_FP32_RECHECK && prec == SJ_PREC_FP64) {
+            set_error("many-join: instance %d: the FP32 margin is not usable (d=%d, k=%d)", i, x.d, x.k);
+            return SJ_ERR_UNSUPPORTED;
         }
         I.prec = prec;
-        if (prec == SJ_PREC_FP64) a.pts32 = nullptr;
+        if (prec == SJ_PREC_FP64) {
+            a.pts32 = nullptr;
+            a.mDP = 0;
+        }
         SJ_TRY(build_tiles(I.g, a, 0u, (uint32_t)x.n, I.tileStart, I.nTilesDev));
         a.tileStart = I.tileStart.p;
         a.nTilesDev = I.nTilesDev.p;
@@ -281,6 +287,7 @@ sj_status plan(sj_many_s *m, const sj_instance *in, int count) {
     }
     m->N = N;
     m->W = (uint32_t)W;
+    m->mfma = P.precision == SJ_PREC_FP32_RECHECK ? 1 : 0;
     hInst.reserve(W);
     hTile.reserve(W);
     for (int i = 0; i < count; i++)

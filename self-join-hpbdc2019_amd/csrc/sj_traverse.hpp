@@ -31,6 +31,10 @@ struct JoinArgs {
     int d, k, form, useTiles;
     double T;
     float lo32, hi32, T32, pad1;
+    // matrix-core FP32 form (mDP > 0): error bound mAlpha * s + mBeta,
+    // certain hit at or below mTlo, certain miss above mThi
+    float mAlpha, mBeta, mTlo, mThi, mT32;
+    int mDP, mSrc32;  // mSrc32: stage from pts32 (FP32 copy, rows of mDP floats)
     // query range (sorted positions) and tiles of 64 over it
     uint32_t qBegin, qEnd;
     const uint32_t *tileStart;  // tile t = sorted positions [tileStart[t], tileStart[t+1])
@@ -240,33 +244,114 @@ __device__ __forceinline__ double d2_fp64(const double *p, const double *q, int 
 }
 
 // ------------------------------------------------- matrix-core path
-constexpr int MFMA_MIN_D = 16;  // FP32+recheck joins with MFMA_MIN_D <= d <= MFMA_MAX_D
-constexpr int MFMA_MAX_D = 64;  // run on the matrix cores
+constexpr int MFMA_MAX_D = 64;  // FP32+recheck joins with d <= MFMA_MAX_D run on the matrix cores
 
-// Decision constants of the matrix-core distance (see mfma_consts).
-struct MfmaConsts {
-    float alpha, beta;  // error bound alpha * (|q|^2 + |c|^2) + beta
-    float tlo, thi;     // certain hit at or below tlo, certain miss above thi
-    float t32;          // FP32 threshold: the decision before recheck
-    int dp;             // padded dims = row stride of the FP32 copy
-};
-
-// Padded dims of the FP32 copy for the matrix-core path, 0 if d is outside it.
+// Padded dims of the matrix-core operands (4, 8, 16, 32 or 64), 0 above
+// MFMA_MAX_D.
 int mfma_padded_dims(int d);
-MfmaConsts mfma_consts(double T, double Rcoord, int d, int DP, bool *usable);
-sj_status launch_join_mfma(const JoinArgs &a, const MfmaConsts &mc, int pass, int cu_count, hipStream_t s);
+// Fills the matrix-core fields of a (mAlpha .. mSrc32) for threshold T;
+// R >= 0: staging from the FP32 copy whose coordinates are within R of the
+// origin (Euclidean), R < 0: staging centred in FP64.
+void mfma_consts(JoinArgs &a, double T, double R);
+sj_status launch_join_mfma(const JoinArgs &a, int pass, int cu_count, hipStream_t s);
+
+// Own and mirrored hits of one staged chunk of at most 32 candidates at
+// sorted positions [cb, cb + nv).  hit: this lane's hits (bit c =
+// candidate cb + c); mirBits: those whose mirrored pair goes to the
+// candidate's row.  Lid, Lcnt, Lbase: the chunk's IDs, mirrored-hit
+// counters (zero on entry) and reserved bases in LDS.  Own hits leave one
+// source lane at a time, four lanes per round, as contiguous row writes;
+// mirrored hits are counted with one atomic per candidate (count pass) or
+// reserved with one atomic per candidate and placed through the LDS
+// counter (fill pass).
+template <int PASS>
+__device__ __forceinline__ void emit_chunk32(const JoinArgs &a, uint32_t cb, uint32_t nv, uint32_t hit,
+                                             uint32_t mirBits, bool ownW, int32_t myId, uint64_t ownPos,
+                                             uint64_t ownLim, uint32_t &own, const int32_t *Lid, uint32_t *Lcnt,
+                                             unsigned long long *Lbase, bool anyMirror, uint64_t &err) {
+    const unsigned lane = __lane_id();
+    const uint32_t myCnt = (uint32_t)__popc(hit);
+    uint32_t ownBits = 0;
+    if (PASS == PASS_FILL && ownW) {
+        // more hits than the row has room for: count and fill disagree
+        if (ownPos + own + myCnt <= ownLim) ownBits = hit;
+        else err++;
+    }
+    const uint64_t wcur = ownPos + own;
+    const uint32_t lc = min(lane, 31u);
+    uint64_t todo = __ballot(ownBits != 0u || mirBits != 0u);
+    while (todo) {
+        int l[4];
+        bool lv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            lv[u] = todo != 0ull;
+            l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+            todo &= todo - 1ull;
+        }
+        uint32_t oB[4], mB[4];
+        uint64_t w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            oB[u] = lv[u] ? readlane_u32(ownBits, l[u]) : 0u;
+            mB[u] = lv[u] ? readlane_u32(mirBits, l[u]) : 0u;
+            w[u] = readlane_u64(wcur, l[u]) + (uint32_t)__popc(oB[u] & ((1u << lc) - 1u));
+        }
+        const int32_t idv = Lid[lc];
+        if (lane < 32) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (PASS == PASS_FILL && ((oB[u] >> lane) & 1u)) a.ids[w[u]] = idv;
+                if ((mB[u] >> lane) & 1u) atomicAdd(&Lcnt[lane], 1u);
+            }
+        }
+    }
+    if (ownW) own += myCnt;
+    if (!anyMirror) return;
+    wave_lds_sync();
+    if (PASS == PASS_COUNT) {
+        if (lane < nv && Lcnt[lane]) atomicAdd(&a.mirCnt[cb + lane], Lcnt[lane]);
+        return;
+    }
+    if (lane < nv) {
+        const uint32_t v = Lcnt[lane];
+        if (v) {
+            unsigned long long b = atomicAdd(&a.mirCur[cb + lane], (unsigned long long)v);
+            if (b + v > a.offS[cb + lane + 1] - a.outBase) {
+                err++;
+                b = NO_SLOT;
+            }
+            Lbase[lane] = b;
+        }
+        Lcnt[lane] = 0u;
+    }
+    wave_lds_sync();
+    uint64_t todo2 = __ballot(mirBits != 0u);
+    while (todo2) {
+        const int l = __builtin_ctzll(todo2);
+        todo2 &= todo2 - 1ull;
+        const uint32_t mB = readlane_u32(mirBits, l);
+        const int32_t qid = (int32_t)readlane_u32((uint32_t)myId, l);
+        if (lane < 32 && ((mB >> lane) & 1u)) {
+            const uint32_t k2 = atomicAdd(&Lcnt[lane], 1u);
+            const unsigned long long b = Lbase[lane];
+            if (b != NO_SLOT) a.ids[b + k2] = qid;
+        }
+    }
+}
 
 // ------------------------------------------------- host helpers (sj_join.hip)
 double eps_threshold(double eps);
 JoinArgs base_args(sj_grid_s *g, int form, double T);
 sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
                       DBuf<uint32_t> &nTiles);
-sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec, MfmaConsts *mc, const MfmaConsts **mcp);
+sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec);
 
-// One persistent launch over (instance, tile) work items of many joins.
+// One persistent launch over (instance, tile) work items of many joins;
+// mfma: every instance runs the matrix-core FP32 walk, else the FP64 walks.
 sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
-                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, unsigned blocks,
-                           hipStream_t s);
+                           uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, int mfma,
+                           unsigned blocks, hipStream_t s);
 
 // Work-queue keys of tiles [0, bound) of one join (descending 8-bit
 // log-scale bucket of the staged candidates; val[t] = t).

@@ -104,3 +104,18 @@ def test_mfma_fp32_and_fp64_forms_agree(ctx):
     b = ctx.self_join(pts, k, eps, form="lid", precision=sjamd.FP32_RECHECK)
     assert a.total == b.total == int(ref[0][-1])
     _assert_same(b.csr(), (O.csr_rows_sorted(*a.csr())))
+
+
+def test_same_grid_fp32_walks_in_turn(ctx):
+    """one grid joined in FP32 per lane (rows of d floats) and then on the
+    matrix cores (rows padded to 16): the cached FP32 copy grows"""
+    import sjamd
+
+    pts, k, eps, ref = _case("clusters16d_k3")
+    _say("[mfma] same grid, per-point then matrix-core FP32")
+    g = ctx.grid_build(pts, k, eps)
+    for per_point in (True, False, True):
+        r = g.join(form="lid", precision=sjamd.FP32_RECHECK, per_point=per_point)
+        _assert_same(r.csr(), ref)
+        r.free()
+    g.free()
