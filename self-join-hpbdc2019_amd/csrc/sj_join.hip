@@ -209,11 +209,13 @@ struct TileCfg {
     static constexpr int MW = CH / 64;                              // mask words per lane (<= 4)
 };
 
+// Staged rows are padded by 64: a lane reads the 64 rows after its piece
+// start, masking the rows beyond its span.
 template <int D, int PREC>
 struct TileLds {
     using C = TileCfg<D, PREC>;
-    double x64[PREC == 1 ? 1 : C::CH][C::DD];
-    float x32[PREC == 1 ? C::CH : 1][C::DD];
+    double x64[PREC == 1 ? 1 : C::CH + 64][C::DD];
+    float x32[PREC == 1 ? C::CH + 64 : 1][C::DD];
     int32_t id[C::CH];
     uint32_t cnt[C::CH];
     unsigned long long base[C::CH];
@@ -246,8 +248,9 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
     const uint64_t rowBase = linA - c[0];
     const uint64_t rb0 = __shfl(rowBase, firstL, 64);
     if (__ballot(valid && rowBase == rb0) != vmask) {
-        // the tile crosses a cell row: per-lane walk
-        if (valid) process_point<K, D, PASS, PREC>(a, s, st);
+        // the tile crosses a cell row: per-lane walk (FP64: the per-lane
+        // FP32 walk needs the global FP32 copy, which tiles do not make)
+        if (valid) process_point<K, D, PASS, 0>(a, s, st);
         if (PASS == PASS_COUNT && lane == (unsigned)firstL) st.fallback++;
         return;
     }
@@ -264,12 +267,26 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
         p[j] = 0.0;
         p32[j] = 0.0f;
     }
+    // FP32 form: coordinates centred on the tile's first point in FP64 and
+    // rounded once; every staged coordinate is within R of that centre
+    // (R from the tile's span of cells in dim 0), so the band is set per tile
+    double ctr[DD];
+    float lo32 = 0.0f, hi32 = 0.0f, t32 = 0.0f;
+    if constexpr (PREC == 1) {
+#pragma unroll
+        for (int j = 0; j < DD; j++) ctr[j] = a.pts[(uint64_t)ts0 * DD + j];
+        const double R = (double)max(xb - xa + 3u, 4u) * a.cellW * (1.0 + ldexp(1.0, -20));
+        const Fp32Margin m = fp32_margin(a.T, R, D);
+        lo32 = m.usable ? m.lo : -INFINITY;
+        hi32 = m.usable ? m.hi : INFINITY;
+        t32 = m.t32;
+    }
     if (valid) {
 #pragma unroll
         for (int j = 0; j < DD; j++) p[j] = a.pts[(uint64_t)s * DD + j];
         if constexpr (PREC == 1) {
 #pragma unroll
-            for (int j = 0; j < DD; j++) p32[j] = a.pts32[(uint64_t)s * DD + j];
+            for (int j = 0; j < DD; j++) p32[j] = (float)(p[j] - ctr[j]);
         }
     }
     const bool ownW = valid && s >= a.wBegin && s < a.wEnd;
@@ -344,7 +361,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                         const uint64_t t = (uint64_t)c0 + j;
                         if constexpr (PREC == 1) {
 #pragma unroll
-                            for (int q = 0; q < DD; q++) L.x32[j][q] = a.pts32[t * DD + q];
+                            for (int q = 0; q < DD; q++) L.x32[j][q] = (float)(a.pts[t * DD + q] - ctr[q]);
                         } else {
 #pragma unroll
                             for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
@@ -365,67 +382,73 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
                     uint32_t piece = 0;
                     for (uint32_t base = 0; base < maxLen; base += 64, piece++) {
                         const uint32_t n = min(64u, maxLen - base);
-                        uint64_t mask = 0ull;
-                        // uniform trip count; lanes mask candidates outside
-                        // their claimed cells
-                        for (uint32_t i = 0; i < n; i += 4) {
-                            bool ok[4];
-                            uint32_t jj[4];
+                        const uint32_t pstart = clo + base;
+                        // claimed candidates inside this lane's span, as bits
+                        // of the piece; lanes without work read row 0
+                        uint64_t okm = 0ull;
+                        uint32_t r0 = 0u;
+                        if (base < len) {
+                            if (useL) okm |= range_bits(pstart, e0, e1);
+                            if (useC) okm |= range_bits(pstart, e1, e2);
+                            if (useR) okm |= range_bits(pstart, e2, e3);
+                            okm &= range_bits(pstart, pstart, clo + len);
+                            r0 = pstart - c0;
+                        }
+                        // uniform trip count, US candidates per step, one bit
+                        // per candidate; masks are applied per piece
+                        constexpr int US = D <= 3 ? 8 : 4;
+                        uint64_t mask = 0ull, maybe = 0ull;
+                        for (uint32_t i = 0; i < n; i += US) {
+                            uint32_t hb = 0u, mb = 0u;
 #pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                const uint32_t off = base + i + u;
-                                const uint32_t t = clo + off;
-                                const bool in = off < len;
-                                const bool claimed = t < e1 ? useL : (t < e2 ? useC : useR);
-                                ok[u] = in && claimed;
-                                jj[u] = min(t - c0, (uint32_t)(CH - 1));
-                                if (!in) jj[u] = 0;
-                            }
-                            bool h[4];
-                            if constexpr (PREC == 1) {
-                                float q[4][DD];
+                            for (int u = 0; u < US; u++) {
+                                const uint32_t row = r0 + i + (uint32_t)u;
+                                if constexpr (PREC == 1) {
+                                    float q[DD];
 #pragma unroll
-                                for (int u = 0; u < 4; u++)
+                                    for (int z = 0; z < DD; z++) q[z] = L.x32[row][z];
+                                    const float s32 = d2_fp32<D>(p32, q, DD);
+                                    hb |= (s32 < lo32 ? 1u : 0u) << u;
+                                    mb |= (s32 <= hi32 ? 1u : 0u) << u;
+                                } else {
+                                    double q[DD];
 #pragma unroll
-                                    for (int z = 0; z < DD; z++) q[u][z] = L.x32[jj[u]][z];
-                                float s32[4];
-                                bool amb = false;
-#pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    s32[u] = d2_fp32<D>(p32, q[u], DD);
-                                    h[u] = ok[u] && s32[u] < a.lo32;
-                                    amb = amb || (ok[u] && s32[u] >= a.lo32 && s32[u] <= a.hi32);
+                                    for (int z = 0; z < DD; z++) q[z] = L.x64[row][z];
+                                    hb |= (d2_fp64<D>(p, q, DD) <= a.T ? 1u : 0u) << u;
                                 }
-                                if (amb) {
+                            }
+                            mask |= (uint64_t)hb << i;
+                            if constexpr (PREC == 1) maybe |= (uint64_t)mb << i;
+                        }
+                        mask &= okm;
+                        if constexpr (PREC == 1) {
+                            // candidates in [T - M, T + M]: canonical FP64 predicate
+                            const uint64_t bnd = maybe & ~mask & okm;
+                            if (__ballot(bnd != 0ull)) {
+                                uint64_t exm = 0ull, bb = bnd;
+                                while (bb) {
+                                    const int bi = __builtin_ctzll(bb);
+                                    bb &= bb - 1ull;
+                                    const uint64_t t = (uint64_t)pstart + (uint32_t)bi;
+                                    const bool ex = d2_fp64<D>(p, a.pts + t * DD, DD) <= a.T;
+                                    if (ex) exm |= 1ull << bi;
+                                    if (PASS == PASS_COUNT) {
+                                        // the plain FP32 decision, for the count of
+                                        // pairs it would have misclassified
+                                        float q[DD];
 #pragma unroll
-                                    for (int u = 0; u < 4; u++) {
-                                        if (ok[u] && s32[u] >= a.lo32 && s32[u] <= a.hi32) {
-                                            const uint64_t t = (uint64_t)clo + base + i + u;
-                                            h[u] = d2_fp64<D>(p, a.pts + t * DD, DD) <= a.T;
-                                            if (PASS == PASS_COUNT) {
-                                                band++;
-                                                mis += (h[u] != (s32[u] <= a.T32));
-                                            }
-                                        }
+                                        for (int z = 0; z < DD; z++) q[z] = L.x32[r0 + (uint32_t)bi][z];
+                                        mis += (d2_fp32<D>(p32, q, DD) <= t32) != ex ? 1u : 0u;
                                     }
                                 }
-                            } else {
-                                double q[4][DD];
-#pragma unroll
-                                for (int u = 0; u < 4; u++)
-#pragma unroll
-                                    for (int z = 0; z < DD; z++) q[u][z] = L.x64[jj[u]][z];
-#pragma unroll
-                                for (int u = 0; u < 4; u++) h[u] = ok[u] && d2_fp64<D>(p, q[u], DD) <= a.T;
+                                mask |= exm;
+                                if (PASS == PASS_COUNT) band += (uint64_t)__popcll(bnd);
                             }
-#pragma unroll
-                            for (int u = 0; u < 4; u++) mask |= (uint64_t)h[u] << (i + u);
                         }
                         // emission, one lane at a time: the lane's hit mask is
                         // broadcast and each thread takes one candidate bit, so
                         // the lane's hits leave as one contiguous row write;
                         // mirrored hits bump the candidate's LDS counter
-                        const uint32_t pstart = clo + base;
                         uint64_t mirBits = 0ull;
                         if (anyMirror) {
                             uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
@@ -582,8 +605,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, T
 }
 
 template <int K, int D, int PASS, int PREC>
-__global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
-    __shared__ TileLds<D, PREC> lds[WPB];
+__device__ __forceinline__ void join_body(const JoinArgs &a, TileLds<D, PREC> *lds) {
     const unsigned wave = threadIdx.x >> 6;
     const unsigned lane = __lane_id();
     TileLds<D, PREC> &L = lds[wave];
@@ -607,7 +629,7 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
         const uint32_t nbl = (gridDim.x - sl + ns - 1) / ns;  // blocks of this slice
         uint64_t i = (uint64_t)(blockIdx.x / ns) * WPB + wave;
         for (;;) {
-            const uint64_t q = sl + i * ns;
+            const uint64_t q = queue_entry(i, sl, ns);
             if (q >= nTiles) break;
             uint32_t nx = 0;
             if (lane == 0) nx = atomicAdd(a.qctr + sl * QSTRIDE, 1u);
@@ -619,6 +641,20 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
         for (uint32_t tile = blockIdx.x * WPB + wave; tile < nTiles; tile += gridDim.x * WPB) run(tile);
     }
     flush_stats(a, st);
+}
+
+template <int K, int D, int PASS, int PREC>
+__global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
+    __shared__ TileLds<D, PREC> lds[WPB];
+    join_body<K, D, PASS, PREC>(a, lds);
+}
+
+// k <= 2: four waves per SIMD (at most 128 VGPRs); measured faster at
+// 2-D than the wider allocation the compiler picks by itself
+template <int K, int D, int PASS, int PREC>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_join_o4(JoinArgs a) {
+    __shared__ TileLds<D, PREC> lds[WPB];
+    join_body<K, D, PASS, PREC>(a, lds);
 }
 
 // ------------------------------------------------------------ many joins
@@ -960,8 +996,13 @@ using SampleKernel = void (*)(JoinArgs, const uint32_t *, uint64_t);
 
 template <int K, int D>
 static JoinKernel pick_pass(int pass, int prec) {
-    if (pass == PASS_COUNT) return prec ? k_join<K, D, PASS_COUNT, 1> : k_join<K, D, PASS_COUNT, 0>;
-    return prec ? k_join<K, D, PASS_FILL, 1> : k_join<K, D, PASS_FILL, 0>;
+    if constexpr (K <= 2 && D > 0) {
+        if (pass == PASS_COUNT) return prec ? k_join_o4<K, D, PASS_COUNT, 1> : k_join_o4<K, D, PASS_COUNT, 0>;
+        return prec ? k_join_o4<K, D, PASS_FILL, 1> : k_join_o4<K, D, PASS_FILL, 0>;
+    } else {
+        if (pass == PASS_COUNT) return prec ? k_join<K, D, PASS_COUNT, 1> : k_join<K, D, PASS_COUNT, 0>;
+        return prec ? k_join<K, D, PASS_FILL, 1> : k_join<K, D, PASS_FILL, 0>;
+    }
 }
 
 template <int K>
@@ -1055,40 +1096,6 @@ double eps_threshold(double eps) {
 // |sum f^2 - sum a^2| <= E(2 sqrt(D) |a| + D E);
 // FP64 canonical: |d2_64 - |a|^2| <= rho |a|^2, rho = (D+3) u64.
 // d2_64 <= T  =>  s32 <= hi(T/(1-rho));  d2_64 > T  =>  s32 >= lo(T/(1+rho)).
-struct Fp32Margin {
-    bool usable;
-    float lo, hi, t32;
-};
-
-static Fp32Margin fp32_margin(double T, double R, int D) {
-    const double u32 = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
-    const double Rh = R * (1.0 + std::ldexp(1.0, -20)) + std::ldexp(1.0, -126);
-    const double E0 = 2.0 * Rh * (u32 + 2.0 * u64);
-    const double gam = D * u32 / (1.0 - D * u32);
-    const double rho = (D + 3) * u64 * 1.01;
-    const double sD = std::sqrt((double)D);
-    const double tiny = D * std::ldexp(1.0, -126) * 8.0;  // possible FP32 denormal flushes
-    const double Ahi = T / (1.0 - rho);
-    const double nhi = std::sqrt(Ahi);
-    const double Ehi = E0 + u32 * (nhi + E0);
-    const double up = (1.0 + gam) * (Ahi + Ehi * (2.0 * sD * nhi + D * Ehi)) + tiny;
-    const double Alo = T / (1.0 + rho);
-    const double nlo = std::sqrt(Alo);
-    const double Elo = E0 + u32 * (nlo + E0);
-    const double dn = (1.0 - gam) * (Alo - 2.0 * Elo * sD * nlo) - tiny;
-    double M = std::max(up - T, T - dn) * 1.05;
-    Fp32Margin m;
-    m.usable = std::isfinite(M) && M < 0.25 * T && (T + M) < 1e37 && R < 1e30;
-    double lo = T - M, hi = T + M;
-    float flo = (float)lo, fhi = (float)hi;
-    if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
-    if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
-    m.lo = flo;
-    m.hi = fhi;
-    m.t32 = (float)T;
-    return m;
-}
-
 struct EvTimer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
@@ -1136,6 +1143,7 @@ JoinArgs base_args(sj_grid_s *g, int form, double T) {
     a.form = form;
     a.useTiles = 1;
     a.T = T;
+    a.cellW = g->eps;
     a.qBegin = 0;
     a.qEnd = (uint32_t)g->n;
     a.wBegin = 0;
@@ -1321,13 +1329,14 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
 
 sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base);
 
-// Decision constants of the precision form, from an FP32 copy relative to
-// the grid origin and its largest coordinate (one host sync).  With the LDS
-// tiles the matrix cores take d > 8 or d != k (d <= 64): they stage from
-// the FP32 copy when the coordinates lie within 1024 eps of the origin and
-// centre each tile in FP64 otherwise.  The FP32 VALU walk (d == k <= 8, or
-// no tiles) uses the global margin of fp32_margin and falls back to FP64
-// (*prec) when that margin would not be useful.
+// Decision constants of the precision form.  The FP32 LDS tile walk
+// (d == k <= 8) centres each tile in FP64 and sets its band per tile: it
+// needs nothing here.  The others use an FP32 copy relative to the grid
+// origin and its largest coordinate (one host sync): the matrix cores
+// (d > 8 or d != k, d <= 64) stage from that copy when the coordinates lie
+// within 1024 eps of the origin and centre each tile in FP64 otherwise; the
+// per-lane FP32 walk (no tiles) uses the global margin of fp32_margin and
+// falls back to FP64 (*prec) when that margin would not be useful.
 bool use_mfma(const sj_grid_s *g, int useTiles) {
     return useTiles && mfma_padded_dims(g->d) && (g->d > DEV_MAXK || g->d != g->k);
 }
@@ -1340,6 +1349,7 @@ sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec) {
     a.mDP = 0;
     a.mSrc32 = 0;
     const bool mf = use_mfma(g, a.useTiles);
+    if (!mf && a.useTiles && g->d == g->k && g->d <= DEV_MAXK) return SJ_OK;  // tiles centre in FP64
     DBuf<unsigned int> rbits;
     SJ_TRY(rbits.alloc(pool, 1));
     SJ_TRY(fill_u32(s, rbits.p, 1, 0u));

@@ -11,6 +11,7 @@ constexpr int JB = 256;         // traversal block size (4 waves)
 constexpr int WPB = JB / 64;    // waves per block
 constexpr int DEV_MAXK = 8;     // indexed dims handled by the device stencil
 constexpr int QSLICES = 8;      // work-queue slices (one counter each)
+constexpr int QGROUP = 8;       // consecutive queue entries a slice takes together
 constexpr int QSTRIDE = 32;     // u32 words between slice counters (128 B)
 enum { PASS_COUNT = 0, PASS_FILL = 1 };
 enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
@@ -31,6 +32,7 @@ struct JoinArgs {
     int d, k, form, useTiles;
     double T;
     float lo32, hi32, T32, pad1;
+    double cellW;  // grid cell width (eps)
     // matrix-core FP32 form (mDP > 0): error bound mAlpha * s + mBeta,
     // certain hit at or below mTlo, certain miss above mThi
     float mAlpha, mBeta, mTlo, mThi, mT32;
@@ -87,6 +89,13 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
 }
 
 constexpr unsigned long long NO_SLOT = ~0ull;
+
+// Entry i of queue slice sl of ns: slices take groups of QGROUP
+// consecutive entries in turn, so neighbouring tiles (in cell order inside
+// a cost bucket) stay on the XCD that serves the slice.
+__device__ __forceinline__ uint64_t queue_entry(uint64_t i, uint32_t sl, uint32_t ns) {
+    return (sl + (i / QGROUP) * ns) * QGROUP + i % QGROUP;
+}
 
 __device__ __forceinline__ void flush_stats(const JoinArgs &a, const LaneStats &st) {
     const uint64_t c = wave_sum_u64(st.cand), b = wave_sum_u64(st.band), m = wave_sum_u64(st.mis);
@@ -222,6 +231,49 @@ __device__ __forceinline__ bool claims_rt(int form, int k, const uint32_t (&c)[D
     }
     const bool even = (so & 1) == 0;
     return pos != (even && odd);
+}
+
+// Band of the FP32 direct-difference distance (VALU walks).  Stored
+// y = fl32(fl64(x - o)) with every |x_j - o_j| <= R, so
+// |y_j - (x_j - o_j)| <= E0 = 2R(u32 + 2u64); a difference of such values is
+// off by at most E = E0 + u32(|a| + E0).  The FMA sum of D squares is within
+// gamma_D of the sum of squared differences, which is within
+// E(2 sqrt(D)|a| + D E) of |a|^2, and the canonical FP64 sum is within
+// (D+3) u64 |a|^2 of it.  Hence the canonical value is <= T if the FP32 sum
+// is < lo and > T if it is > hi; between lo and hi the FP64 predicate
+// decides.  usable is false when the band would exceed T/4.
+struct Fp32Margin {
+    bool usable;
+    float lo, hi, t32;
+};
+
+__host__ __device__ inline Fp32Margin fp32_margin(double T, double R, int D) {
+    const double u32 = ldexp(1.0, -24), u64 = ldexp(1.0, -53);
+    const double Rh = R * (1.0 + ldexp(1.0, -20)) + ldexp(1.0, -126);
+    const double E0 = 2.0 * Rh * (u32 + 2.0 * u64);
+    const double gam = D * u32 / (1.0 - D * u32);
+    const double rho = (D + 3) * u64 * 1.01;
+    const double sD = sqrt((double)D);
+    const double tiny = D * ldexp(1.0, -126) * 8.0;  // possible FP32 denormal flushes
+    const double Ahi = T / (1.0 - rho);
+    const double nhi = sqrt(Ahi);
+    const double Ehi = E0 + u32 * (nhi + E0);
+    const double up = (1.0 + gam) * (Ahi + Ehi * (2.0 * sD * nhi + D * Ehi)) + tiny;
+    const double Alo = T / (1.0 + rho);
+    const double nlo = sqrt(Alo);
+    const double Elo = E0 + u32 * (nlo + E0);
+    const double dn = (1.0 - gam) * (Alo - 2.0 * Elo * sD * nlo) - tiny;
+    const double M = fmax(up - T, T - dn) * 1.05;
+    Fp32Margin m;
+    m.usable = __builtin_isfinite(M) && M < 0.25 * T && (T + M) < 1e37 && R < 1e30;
+    const double lo = T - M, hi = T + M;
+    float flo = (float)lo, fhi = (float)hi;
+    if ((double)flo > lo) flo = nextafterf(flo, -INFINITY);
+    if ((double)fhi < hi) fhi = nextafterf(fhi, INFINITY);
+    m.lo = flo;
+    m.hi = fhi;
+    m.t32 = (float)T;
+    return m;
 }
 
 // FP64 canonical predicate: dimension order, explicit fma.

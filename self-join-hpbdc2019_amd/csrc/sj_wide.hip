@@ -18,7 +18,7 @@ __global__ __launch_bounds__(JB) void k_join_wide(JoinArgs a) {
         const uint32_t nbl = (gridDim.x - sl + ns - 1) / ns;
         uint64_t i = (uint64_t)(blockIdx.x / ns) * WPB + wave;
         for (;;) {
-            const uint64_t q = sl + i * ns;
+            const uint64_t q = queue_entry(i, sl, ns);
             if (q >= nTiles) break;
             uint32_t nx = 0;
             if (lane == 0) nx = atomicAdd(a.qctr + sl * QSTRIDE, 1u);
