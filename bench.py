@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="timed steps of the plain FP64 form (default: --steps; 1 for clusters64d)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-record", action="store_true",
+                    help="fill evaluates every distance again instead of replaying the count's hit record")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: a 512 MiB device copy, then one FP32+recheck and one FP64 join "
@@ -210,7 +212,7 @@ def bench_single(a, torch):
         ctx = sjamd.Context(0, stream=stream)
         x = ctx.to_device(pts)
         torch.cuda.synchronize()
-        kw = dict(form=form, schedule=sched)
+        kw = dict(form=form, schedule=sched, hit_record=not a.no_record)
         if spec["rounds"]:
             r = ctx.self_join(x, k, eps, **kw)
             kw["capacity"] = r.total // spec["rounds"] + 1
@@ -272,7 +274,7 @@ def bench_single(a, torch):
                          "ms_count": st["ms_count"], "ms_scan": st["ms_scan"], "ms_fill": st["ms_fill"],
                          "band_pairs": st["fp32_band_pairs"],
                          "misclassified_before_recheck": st["fp32_misclassified"],
-                         "candidates": st["candidates"]},
+                         "candidates": st["candidates"], "fill_from_record": st["fill_from_record"]},
         "fp32_speedup_vs_fp64": fp64[0] / dt,
         "roofline": roof,
     }

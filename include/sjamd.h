@@ -66,7 +66,10 @@ typedef struct sj_join_params {
     int32_t block_size;   /* threads per block of the traversal kernels (0: 256) */
     int32_t queue_chunk;  /* query points per work-queue grab (0: 64) */
     int32_t cells_per_tile; /* reserved for tiled traversal (0: auto) */
-    int32_t reserved;
+    int32_t hit_record;   /* 0: the fill replays the hit masks the count pass
+                           * recorded, when they fit the context's record
+                           * space (which grows to the demand seen); -1: the
+                           * fill evaluates every distance again */
 } sj_join_params;
 
 typedef struct sj_grid_info {
@@ -96,6 +99,7 @@ typedef struct sj_stats {
     double ms_fill;              /* device time, fill pass (all batches) */
     double ms_sort_rows;         /* device time, canonical row sort */
     double ms_total;             /* wall time of the call */
+    uint64_t fill_from_record;   /* 1: the fill replayed the count pass's hit record */
 } sj_stats;
 
 /* Row of the reference neighbour table (structs.h:41-47): neighbours of

@@ -35,11 +35,11 @@ sj_status device_as_batch(sj_result r, sj_batch_host &B) {
     const uint64_t n = r->n;
     B.rowOffsets.resize(n + 1);
     B.rowIds.resize(n);
-    B.ids.resize(r->total);
+    SJ_TRY(B.ids.alloc(&r->ctx->hpool, r->total));
     SJ_TRY_HIP(hipStreamSynchronize(r->ctx->stream));
     SJ_TRY_HIP(hipMemcpy(B.rowOffsets.data(), r->d_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost));
     SJ_TRY_HIP(hipMemcpy(B.rowIds.data(), r->d_rowIds.p, n * 4, hipMemcpyDeviceToHost));
-    if (r->total) SJ_TRY_HIP(hipMemcpy(B.ids.data(), r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost));
+    if (r->total) SJ_TRY_HIP(hipMemcpy(B.ids.p, r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost));
     return SJ_OK;
 }
 
@@ -65,8 +65,7 @@ sj_status ensure_host_csr(sj_result r, std::vector<uint64_t> &off, std::vector<i
     off[r->n] = s;
     for (auto &B : r->batches)
         for (size_t q = 0; q < B.rowIds.size(); q++)
-            std::copy(B.ids.begin() + B.rowOffsets[q], B.ids.begin() + B.rowOffsets[q + 1],
-                      ids.begin() + off[B.rowIds[q]]);
+            std::copy(B.ids.p + B.rowOffsets[q], B.ids.p + B.rowOffsets[q + 1], ids.begin() + off[B.rowIds[q]]);
     return SJ_OK;
 }
 
@@ -134,6 +133,10 @@ SJ_API sj_status sj_context_set_stream(sj_context ctx, void *stream) {
 SJ_API void sj_context_destroy(sj_context ctx) {
     if (!ctx) return;
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copyStream) {
+        (void)hipStreamSynchronize(ctx->copyStream);
+        (void)hipStreamDestroy(ctx->copyStream);
+    }
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -339,7 +342,7 @@ SJ_API sj_status sj_result_neighbor_table(sj_result r, sj_neighbor_lookup *table
     }
     for (size_t b = 0; b < r->batches.size(); b++) {
         const auto &B = r->batches[b];
-        if (B.ids.size() > 0x7fffffffull) {
+        if (B.ids.n > 0x7fffffffull) {
             set_error("neighbor table: batch above 2^31-1 pairs");
             return SJ_ERR_OVERFLOW;
         }
@@ -363,8 +366,8 @@ SJ_API sj_status sj_result_batch_data(sj_result r, uint64_t batch, const int32_t
         set_error("sj_result_batch_data: batch %llu of %zu", (unsigned long long)batch, r->batches.size());
         return SJ_ERR_INVALID;
     }
-    *ids = r->batches[batch].ids.data();
-    *count = r->batches[batch].ids.size();
+    *ids = r->batches[batch].ids.p;
+    *count = r->batches[batch].ids.n;
     return SJ_OK;
 }
 

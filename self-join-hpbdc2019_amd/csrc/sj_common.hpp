@@ -46,15 +46,18 @@ void set_error(const char *fmt, ...);
 
 // ------------------------------------------------------------ device pool
 // Caching allocator: power-of-two size classes, freed blocks are kept for
-// reuse so steady-state joins make no hipMalloc calls.
+// reuse so steady-state joins make no hipMalloc calls.  A pinned pool
+// hands out page-locked host memory (hipHostMalloc) instead.
 class Pool {
   public:
+    explicit Pool(bool pinned = false) : pinned_(pinned) {}
     ~Pool();
     sj_status alloc(size_t bytes, void **out);
     void release(void *p);
     void trim();
 
   private:
+    bool pinned_;
     std::mutex mu_;
     std::map<size_t, std::vector<void *>> free_;
     std::map<void *, size_t> live_;
@@ -140,8 +143,13 @@ struct sj_context_s {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     sj::Pool pool;
+    sj::Pool hpool{true};        // pinned host memory: result batches, staging
+    hipStream_t copyStream = nullptr;  // result-batch downloads (created on first use)
     uint64_t *pinned = nullptr;  // 64 scalars of pinned readback space
     int cu_count = 256;
+    // hit-record space of the joins on this context (directory entries,
+    // mask words): grows to the demand a join reports when it did not fit
+    uint64_t recDirCap = 0, recMaskCap = 0;
 };
 
 struct sj_grid_s {
@@ -166,7 +174,7 @@ struct sj_grid_s {
 };
 
 struct sj_batch_host {
-    std::vector<int32_t> ids;          // neighbour IDs of the batch rows
+    sj::DBuf<int32_t> ids;             // neighbour IDs of the batch rows (pinned host)
     std::vector<uint32_t> rowIds;      // original IDs of the rows
     std::vector<uint64_t> rowOffsets;  // rows+1, into ids
 };

@@ -28,10 +28,15 @@ static size_t size_class(size_t b) {
     return c;
 }
 
+static void pool_free(bool pinned, void *p) {
+    if (pinned) (void)hipHostFree(p);
+    else (void)hipFree(p);
+}
+
 Pool::~Pool() {
     for (auto &kv : free_)
-        for (void *p : kv.second) (void)hipFree(p);
-    for (auto &kv : live_) (void)hipFree(kv.first);
+        for (void *p : kv.second) pool_free(pinned_, p);
+    for (auto &kv : live_) pool_free(pinned_, kv.first);
 }
 
 sj_status Pool::alloc(size_t bytes, void **out) {
@@ -45,17 +50,18 @@ sj_status Pool::alloc(size_t bytes, void **out) {
         return SJ_OK;
     }
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, c);
+    auto get = [&]() { return pinned_ ? hipHostMalloc(&p, c, hipHostMallocDefault) : hipMalloc(&p, c); };
+    hipError_t e = get();
     if (e != hipSuccess) {
         // release cached blocks and retry once
         (void)hipGetLastError();
         for (auto &kv : free_)
-            for (void *q : kv.second) (void)hipFree(q);
+            for (void *q : kv.second) pool_free(pinned_, q);
         free_.clear();
-        e = hipMalloc(&p, c);
+        e = get();
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            set_error("device allocation of %zu bytes failed", c);
+            set_error("%s allocation of %zu bytes failed", pinned_ ? "pinned host" : "device", c);
             return SJ_ERR_NOMEM;
         }
     }
@@ -75,7 +81,7 @@ void Pool::release(void *p) {
 void Pool::trim() {
     std::lock_guard<std::mutex> lk(mu_);
     for (auto &kv : free_)
-        for (void *p : kv.second) (void)hipFree(p);
+        for (void *p : kv.second) pool_free(pinned_, p);
     free_.clear();
 }
 

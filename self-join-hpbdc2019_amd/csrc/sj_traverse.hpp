@@ -53,6 +53,33 @@ struct JoinArgs {
     uint64_t outBase;
     unsigned long long *mirCur;
     int32_t *ids;
+    // hit record (LDS tile walk): the count pass stores, per executed
+    // 64-candidate piece, a directory entry {lanes with hits, first mask
+    // word} and the hit masks of those lanes back to back; the fill replays
+    // them instead of evaluating distances again.  recDir null: no record.
+    uint64_t *recDir;              // 2 words per entry
+    uint64_t *recMask;
+    const uint64_t *tileDir;       // tile t: entries [tileDir[t], tileDir[t+1])
+    unsigned long long *recCtr;    // REC_* counters
+    uint64_t dirCap, maskCap;      // entries, words
+};
+
+// Hit-record counters: record unusable (any cause), then REC_NSLICE
+// slices REC_STRIDE words apart, each {blocks taken, mask words wanted}.  A
+// wave takes blocks of REC_BLK mask words from slice (wave id mod
+// REC_NSLICE); block b of slice c is global block b * REC_NSLICE + c.
+enum { REC_BAD = 0, REC_SLICE0 = 32 };
+constexpr uint32_t REC_BLK = 512;
+constexpr uint32_t REC_NSLICE = 64;
+constexpr uint32_t REC_STRIDE = 32;
+constexpr uint32_t REC_NCTR = REC_SLICE0 + REC_NSLICE * REC_STRIDE;
+
+// Per-wave state of the hit record: the wave's current block of mask words.
+struct TileRec {
+    uint64_t blk = 0;
+    uint32_t used = REC_BLK;
+    bool ok = true;
+    uint64_t want = 0;
 };
 
 __device__ __forceinline__ uint32_t popc_below64(uint64_t m) {

@@ -54,7 +54,7 @@ class JoinParams(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("precision", C.c_int32), ("schedule", C.c_int32),
         ("sort_rows", C.c_int32), ("capacity", C.c_uint64), ("block_size", C.c_int32),
-        ("queue_chunk", C.c_int32), ("cells_per_tile", C.c_int32), ("reserved", C.c_int32),
+        ("queue_chunk", C.c_int32), ("cells_per_tile", C.c_int32), ("hit_record", C.c_int32),
     ]
 
 
@@ -74,7 +74,7 @@ class Stats(C.Structure):
         ("overflow_retries", C.c_uint64),
         ("ms_grid", C.c_double), ("ms_workload", C.c_double), ("ms_count", C.c_double),
         ("ms_scan", C.c_double), ("ms_fill", C.c_double), ("ms_sort_rows", C.c_double),
-        ("ms_total", C.c_double),
+        ("ms_total", C.c_double), ("fill_from_record", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -226,14 +226,17 @@ def device_copy(ptr, count, torch_dtype, device=0):
 
 
 def make_params(form=LID, precision=FP64, schedule=STRIDED, sort_rows=False, capacity=0,
-                queue_chunk=0, per_point=False):
+                queue_chunk=0, per_point=False, hit_record=True):
     """per_point=True disables the LDS tile walk (every lane walks its
-    neighbour cells from memory); results are identical."""
+    neighbour cells from memory); hit_record=False makes the fill evaluate
+    every distance again instead of replaying the count pass's hit masks.
+    Results are identical either way."""
     if isinstance(form, str):
         form = FORMS[form]
     return JoinParams(form=form, precision=precision, schedule=schedule,
                       sort_rows=1 if sort_rows else 0, capacity=capacity, block_size=0,
-                      queue_chunk=queue_chunk, cells_per_tile=-1 if per_point else 0, reserved=0)
+                      queue_chunk=queue_chunk, cells_per_tile=-1 if per_point else 0,
+                      hit_record=0 if hit_record else -1)
 
 
 class Context:
