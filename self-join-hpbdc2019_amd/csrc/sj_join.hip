@@ -319,12 +319,13 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     const uint32_t nC0 = a.nCells[0];
     const uint32_t x = c[0];
 
-    // query point in registers
-    double p[DD];
+    // query point in registers (FP32 form: the FP32 copy only; the FP64
+    // coordinates are read again for the rare band recheck)
+    double p[PREC == 1 ? 1 : DD];
     float p32[DD];
 #pragma unroll
     for (int j = 0; j < DD; j++) {
-        p[j] = 0.0;
+        if constexpr (PREC != 1) p[j] = 0.0;
         p32[j] = 0.0f;
     }
     // FP32 form: coordinates centred on the tile's first point in FP64 and
@@ -334,7 +335,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     float lo32 = 0.0f, hi32 = 0.0f, t32 = 0.0f;
     if constexpr (PREC == 1 && !RP) {
 #pragma unroll
-        for (int j = 0; j < DD; j++) ctr[j] = a.pts[(uint64_t)ts0 * DD + j];
+        for (int j = 0; j < DD; j++) ctr[j] = uniform_f64(a.pts[(uint64_t)ts0 * DD + j]);
         const double Rt = (double)max(xb - xa + 3u, 4u) * a.cellW * (1.0 + ldexp(1.0, -20));
         const Fp32Margin m = fp32_margin(a.T, Rt, D);
         lo32 = m.usable ? m.lo : -INFINITY;
@@ -342,11 +343,12 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
         t32 = m.t32;
     }
     if (valid && !RP) {
-#pragma unroll
-        for (int j = 0; j < DD; j++) p[j] = a.pts[(uint64_t)s * DD + j];
         if constexpr (PREC == 1) {
 #pragma unroll
-            for (int j = 0; j < DD; j++) p32[j] = (float)(p[j] - ctr[j]);
+            for (int j = 0; j < DD; j++) p32[j] = (float)(a.pts[(uint64_t)s * DD + j] - ctr[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < DD; j++) p[j] = a.pts[(uint64_t)s * DD + j];
         }
     }
     // hit record: entries [tileDir[tile], tileDir[tile+1]) hold the tile's
@@ -532,7 +534,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                                     const int bi = __builtin_ctzll(bb);
                                     bb &= bb - 1ull;
                                     const uint64_t t = (uint64_t)pstart + (uint32_t)bi;
-                                    const bool ex = d2_fp64<D>(p, a.pts + t * DD, DD) <= a.T;
+                                    const bool ex = d2_fp64<D>(a.pts + (uint64_t)s * DD, a.pts + t * DD, DD) <= a.T;
                                     if (ex) exm |= 1ull << bi;
                                     if (PASS == PASS_COUNT) {
                                         // the plain FP32 decision, for the count of
@@ -798,6 +800,15 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
 // 2-D than the wider allocation the compiler picks by itself
 template <int K, int D, int PASS, int PREC>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_join_o4(JoinArgs a) {
+    __shared__ TileLds<D, PREC> lds[WPB];
+    join_body<K, D, PASS, PREC, false>(a, lds);
+}
+
+// 5 <= k <= 7: three waves per SIMD (at most 168 VGPRs), which the LDS
+// of these tiles still allows; measured faster at 6-D than the two waves
+// the compiler's own allocation leaves
+template <int K, int D, int PASS, int PREC>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(3))) void k_join_o3(JoinArgs a) {
     __shared__ TileLds<D, PREC> lds[WPB];
     join_body<K, D, PASS, PREC, false>(a, lds);
 }
@@ -1170,6 +1181,9 @@ static JoinKernel pick_pass(int pass, int prec) {
     if constexpr (K <= 2 && D > 0) {
         if (pass == PASS_COUNT) return prec ? k_join_o4<K, D, PASS_COUNT, 1> : k_join_o4<K, D, PASS_COUNT, 0>;
         return prec ? k_join_o4<K, D, PASS_FILL, 1> : k_join_o4<K, D, PASS_FILL, 0>;
+    } else if constexpr (K >= 5 && K <= 7 && D > 0) {
+        if (pass == PASS_COUNT) return prec ? k_join_o3<K, D, PASS_COUNT, 1> : k_join_o3<K, D, PASS_COUNT, 0>;
+        return prec ? k_join_o3<K, D, PASS_FILL, 1> : k_join_o3<K, D, PASS_FILL, 0>;
     } else {
         if (pass == PASS_COUNT) return prec ? k_join<K, D, PASS_COUNT, 1> : k_join<K, D, PASS_COUNT, 0>;
         return prec ? k_join<K, D, PASS_FILL, 1> : k_join<K, D, PASS_FILL, 0>;
@@ -1845,12 +1859,13 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             const int sl = (int)(bi & 1);
             const uint32_t s0 = win[bi].first, s1 = win[bi].second;
             // query range: the window's cells, widened to every cell that may
-            // claim one of them (adjacent cells lie within Mmax linear IDs)
+            // claim one of them (adjacent cells lie within Mmax linear IDs;
+            // LID cells claim only larger linear IDs)
             uint64_t q0 = s0, q1 = s1;
             if (P->form != SJ_FORM_FULL) {
                 const uint64_t linLo = hLin.p[hCellOf.p[s0]], linHi = hLin.p[hCellOf.p[s1 - 1]];
                 const uint64_t lo = linLo > Mmax ? linLo - Mmax : 0;
-                const uint64_t hi = linHi + Mmax;
+                const uint64_t hi = P->form == SJ_FORM_LID ? linHi : linHi + Mmax;
                 const uint64_t c0 = std::lower_bound(hLin.p, hLin.p + nne, lo) - hLin.p;
                 const uint64_t c1 = std::upper_bound(hLin.p, hLin.p + nne, hi) - hLin.p;
                 q0 = hStart.p[c0];
