@@ -167,8 +167,7 @@ struct sj_grid_s {
     sj::DBuf<uint32_t> cellStart;    // nNonEmpty + 1
     sj::DBuf<uint64_t> cellLinear;   // nNonEmpty
     sj::DBuf<uint32_t> cellCoord;    // nNonEmpty x k
-    sj::DBuf<int32_t> denseCell;     // totalCells (dense lookup) or empty
-    sj::DBuf<uint32_t> posLB;        // totalCells+1: first sorted position with linear ID >= L
+    sj::DBuf<uint32_t> posLB;        // totalCells+1 (dense grids): first sorted position with linear ID >= L
     sj::DBuf<float> sortedPts32;     // lazily built for the FP32 form
     double ms_build = 0.0;
 };

@@ -208,25 +208,26 @@ __global__ __launch_bounds__(GB) void k_cell_coords(const double *spts, const ui
     for (int j = 0; j < k; j++) coord[c * k + j] = (uint32_t)((p[j] - gp->minArr[j]) / eps);
 }
 
-__global__ __launch_bounds__(GB) void k_dense_scatter(const uint64_t *cellLinear, uint64_t nne,
-                                                      int32_t *dense) {
-    uint64_t c = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    if (c >= nne) return;
-    dense[cellLinear[c]] = (int32_t)c;
-}
-
-__global__ __launch_bounds__(GB) void k_occupied(const int32_t *dense, uint64_t tc, uint32_t *occ) {
-    uint64_t L = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    if (L >= tc) return;
-    occ[L] = dense[L] >= 0 ? 1u : 0u;
-}
-
-// posLB[L] = cellStart[#non-empty cells with linear ID < L], L in [0, tc]
-__global__ __launch_bounds__(GB) void k_pos_lb(const uint32_t *lb, const uint32_t *cellStart, uint64_t tc,
-                                               uint32_t *posLB) {
-    uint64_t L = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    if (L > tc) return;
-    posLB[L] = cellStart[lb[L]];
+// posLB[L], L in [0, tc]: the first sorted position whose cell has linear
+// ID >= L, i.e. cellStart of the first non-empty cell at or after L (n past
+// the last).  A cell L is non-empty iff posLB[L] < posLB[L + 1].  Thread t
+// fills the run [t * POSLB_RUN, (t + 1) * POSLB_RUN): one binary search of
+// the sorted cell IDs, then a forward walk.
+constexpr uint32_t POSLB_RUN = 16;
+__global__ __launch_bounds__(GB) void k_pos_lb(const uint64_t *cellLinear, const uint32_t *cellStart,
+                                               uint64_t nne, uint64_t tc, uint32_t *posLB) {
+    const uint64_t L0 = ((uint64_t)blockIdx.x * GB + threadIdx.x) * POSLB_RUN;
+    if (L0 > tc) return;
+    uint64_t lo = 0, hi = nne;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (cellLinear[mid] < L0) lo = mid + 1; else hi = mid;
+    }
+    const uint64_t L1 = L0 + POSLB_RUN < tc + 1 ? L0 + POSLB_RUN : tc + 1;
+    for (uint64_t L = L0; L < L1; L++) {
+        while (lo < nne && cellLinear[lo] < L) lo++;
+        posLB[L] = cellStart[lo];
+    }
 }
 
 static unsigned blocks_for(uint64_t n, unsigned b = GB) {
@@ -354,25 +355,14 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
                        g->cellStart.p, nne, d, k, eps, g->d_gp.p, g->cellCoord.p);
     SJ_LAUNCH_CHECK();
 
-    // O(1) lookup when the full cell table is at most 64M entries or 16x n
+    // O(1) lookups when the linear-ID table is at most 64M entries or 16x n:
+    // posLB gives both a cell's sorted range and whether it is non-empty
     const uint64_t tc = g->gp.totalCells;
     g->dense = tc <= (1ull << 26) || tc <= 16ull * n;
     if (g->dense) {
-        SJ_TRY(g->denseCell.alloc(pool, tc));
-        SJ_TRY(fill_u32(s, reinterpret_cast<uint32_t *>(g->denseCell.p), tc, 0xFFFFFFFFu));
-        hipLaunchKernelGGL(k_dense_scatter, dim3(blocks_for(nne)), dim3(GB), 0, s, g->cellLinear.p, nne,
-                           g->denseCell.p);
-        SJ_LAUNCH_CHECK();
-        // row-range lookup: first sorted position at or after each linear ID
-        DBuf<uint32_t> occ, lb;
-        SJ_TRY(occ.alloc(pool, tc));
-        SJ_TRY(lb.alloc(pool, tc + 1));
-        hipLaunchKernelGGL(k_occupied, dim3(blocks_for(tc)), dim3(GB), 0, s, g->denseCell.p, tc, occ.p);
-        SJ_LAUNCH_CHECK();
-        SJ_TRY(scan_exclusive_u32_to_u32(pool, s, occ.p, tc, lb.p));
         SJ_TRY(g->posLB.alloc(pool, tc + 1));
-        hipLaunchKernelGGL(k_pos_lb, dim3(blocks_for(tc + 1)), dim3(GB), 0, s, lb.p, g->cellStart.p, tc,
-                           g->posLB.p);
+        hipLaunchKernelGGL(k_pos_lb, dim3(blocks_for(tc / POSLB_RUN + 1)), dim3(GB), 0, s, g->cellLinear.p,
+                           g->cellStart.p, nne, tc, g->posLB.p);
         SJ_LAUNCH_CHECK();
     }
     return SJ_OK;

@@ -1331,7 +1331,6 @@ JoinArgs base_args(sj_grid_s *g, int form, double T) {
     a.cellStart = g->cellStart.p;
     a.cellLinear = g->cellLinear.p;
     a.cellCoord = g->cellCoord.p;
-    a.dense = g->dense ? g->denseCell.p : nullptr;
     a.posLB = g->dense ? g->posLB.p : nullptr;
     a.nne = g->nNonEmpty;
     for (int j = 0; j < DEV_MAXK; j++) {

@@ -24,7 +24,6 @@ struct JoinArgs {
     const uint32_t *cellStart;
     const uint64_t *cellLinear;
     const uint32_t *cellCoord;
-    const int32_t *dense;
     const uint32_t *posLB;    // dense grids: first position with linear ID >= L
     uint64_t nne;
     uint32_t nCells[DEV_MAXK];
@@ -150,7 +149,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 __device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
-    if (a.dense) return a.dense[lin];
+    if (a.posLB) {
+        const uint32_t p0 = a.posLB[lin];
+        return p0 < a.posLB[lin + 1] ? (int64_t)a.cellOf[p0] : -1;
+    }
     uint64_t lo = 0, hi = a.nne;
     while (lo < hi) {
         uint64_t mid = (lo + hi) >> 1;
