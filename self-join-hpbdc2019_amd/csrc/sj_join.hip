@@ -310,8 +310,12 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     if (__ballot(valid && rowBase == rb0) != vmask) {
         // the tile crosses a cell row: per-lane walk (FP64: the per-lane
         // FP32 walk needs the global FP32 copy, which tiles do not make)
+        // a record cannot describe it: the count marks the record unusable
         if (valid) process_point<K, D, PASS, 0>(a, s, st);
-        if (PASS == PASS_COUNT && lane == (unsigned)firstL) st.fallback++;
+        if (PASS == PASS_COUNT && lane == (unsigned)firstL) {
+            st.fallback++;
+            if (a.recDir) atomicAdd(&a.recCtr[REC_BAD], 1ull);
+        }
         return;
     }
     const uint32_t xa = __shfl(c[0], firstL, 64);
@@ -442,13 +446,14 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     uint64_t rm[Cfg::MW];
 #pragma unroll
                     for (int w = 0; w < Cfg::MW; w++) rm[w] = 0ull;
+                    const uint32_t np = (maxLen + 63u) / 64u;  // pieces of the chunk
+                    const uint64_t chunkIt = piece_it;
                     if constexpr (RP) {
-                        const uint32_t np = (maxLen + 63u) / 64u;
                         uint64_t anyHit = 0ull;
 #pragma unroll
                         for (int w = 0; w < Cfg::MW; w++) {
                             if ((uint32_t)w < np) {
-                                const uint64_t *e = a.recDir + 2 * (dbase + piece_it + (uint64_t)w);
+                                const uint64_t *e = a.recDir + REC_DW * (dbase + piece_it + (uint64_t)w);
                                 const uint64_t pres = e[0];
                                 anyHit |= pres;
                                 if ((pres >> lane) & 1ull) rm[w] = a.recMask[e[1] + popc_below64(pres)];
@@ -458,6 +463,32 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                         if (!anyHit) {
                             piece_it += np;
                             continue;
+                        }
+                        // mirrored hits: each candidate's range comes from the
+                        // reservation the count pass recorded, packed as
+                        // (row position << 7) | hits
+                        const uint64_t mo = anyMirror ? a.recDir[REC_DW * (dbase + piece_it) + 2] : 0ull;
+                        if (mo) {
+                            const uint64_t *mr = a.recMask + (mo - 1);
+                            uint32_t pre = Cfg::MW;
+#pragma unroll
+                            for (int w = 0; w < Cfg::MW; w++) {
+                                const uint64_t hv = mr[w];
+                                if ((hv >> lane) & 1ull) {
+                                    const uint64_t rv = mr[pre + popc_below64(hv)];
+                                    const uint32_t j = lane + 64u * (uint32_t)w;
+                                    const uint64_t row = (uint64_t)c0 + j;
+                                    unsigned long long bs = NO_SLOT;
+                                    if (row >= a.wBegin && row < a.wEnd) {
+                                        const uint64_t v = rv & 255ull;
+                                        const uint64_t b0 = a.offS[row] - a.outBase + a.ownCnt[row] + (rv >> 8);
+                                        if (b0 + v <= a.offS[row + 1] - a.outBase) bs = (b0 << 7) | v;
+                                        else err++;
+                                    }
+                                    L.base[j] = bs;
+                                }
+                                pre += (uint32_t)__popcll(hv);
+                            }
                         }
                     }
                     // stage the chunk
@@ -577,8 +608,11 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                                     if (mask) a.recMask[off + popc_below64(pres)] = mask;
                                     R.used += nz;
                                     if (lane == 0) {
-                                        a.recDir[2 * (dbase + piece_it)] = pres;
-                                        a.recDir[2 * (dbase + piece_it) + 1] = off;
+                                        uint64_t *e = a.recDir + REC_DW * (dbase + piece_it);
+                                        e[0] = pres;
+                                        e[1] = off;
+                                        e[2] = 0ull;
+                                        e[3] = 0ull;
                                     }
                                 }
                             }
@@ -636,7 +670,18 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
                                     const uint64_t mB = lv[u] ? readlane_u64(mirBits, l[u]) : 0ull;
-                                    if ((mB >> lane) & 1ull) atomicAdd(&L.cnt[j[u]], 1u);
+                                    if constexpr (RP) {
+                                        // placed now: slot k2 of the recorded range
+                                        const int32_t sid = (int32_t)readlane_u32((uint32_t)myId, l[u]);
+                                        if ((mB >> lane) & 1ull) {
+                                            const uint32_t k2 = atomicAdd(&L.cnt[j[u]], 1u);
+                                            const unsigned long long bs = L.base[j[u]];
+                                            if (bs != NO_SLOT && k2 < (uint32_t)(bs & 127ull)) a.ids[(bs >> 7) + k2] = sid;
+                                            else if (bs != NO_SLOT) err++;
+                                        }
+                                    } else {
+                                        if ((mB >> lane) & 1ull) atomicAdd(&L.cnt[j[u]], 1u);
+                                    }
                                 }
                             }
                         }
@@ -652,11 +697,45 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     wave_lds_sync();
                     if (anyMirror) {
                         if (PASS == PASS_COUNT) {
-                            for (uint32_t j = lane; j < m; j += 64) {
-                                const uint32_t v = L.cnt[j];
-                                if (v) atomicAdd(&a.mirCnt[c0 + j], v);
+                            // reserve the chunk's mirrored hits of each candidate;
+                            // a record keeps each reservation for the fill
+                            uint64_t hv[Cfg::MW], rv[Cfg::MW];
+                            uint32_t nzm = Cfg::MW;
+#pragma unroll
+                            for (int w = 0; w < Cfg::MW; w++) {
+                                const uint32_t j = lane + 64u * (uint32_t)w;
+                                const uint32_t v = j < m ? L.cnt[j] : 0u;
+                                uint32_t b = 0;
+                                if (v) b = atomicAdd(&a.mirCnt[c0 + j], v);
+                                hv[w] = __ballot(v != 0u);
+                                rv[w] = ((uint64_t)b << 8) | v;
+                                nzm += (uint32_t)__popcll(hv[w]);
                             }
-                        } else {
+                            if (recOn && rec && R.ok && np > 0 && nzm > (uint32_t)Cfg::MW) {
+                                if (R.used + nzm > REC_BLK) {
+                                    const uint32_t sl = (blockIdx.x * WPB + (threadIdx.x >> 6)) % REC_NSLICE;
+                                    uint64_t b = 0;
+                                    if (lane == 0) b = atomicAdd(&a.recCtr[REC_SLICE0 + sl * REC_STRIDE], 1ull);
+                                    R.blk = (readlane_u64(b, 0) * REC_NSLICE + sl) * REC_BLK;
+                                    R.used = 0;
+                                    R.ok = R.blk + REC_BLK <= a.maskCap;
+                                    if (!R.ok && lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
+                                }
+                                if (R.ok) {
+                                    const uint64_t off = R.blk + R.used;
+                                    uint32_t pre = Cfg::MW;
+#pragma unroll
+                                    for (int w = 0; w < Cfg::MW; w++) {
+                                        if (lane == (unsigned)w) a.recMask[off + w] = hv[w];
+                                        if ((hv[w] >> lane) & 1ull) a.recMask[off + pre + popc_below64(hv[w])] = rv[w];
+                                        pre += (uint32_t)__popcll(hv[w]);
+                                    }
+                                    R.used += nzm;
+                                    if (lane == 0) a.recDir[REC_DW * (dbase + chunkIt) + 2] = off + 1;
+                                }
+                            }
+                            R.want += nzm > (uint32_t)Cfg::MW ? nzm : 0u;
+                        } else if constexpr (!RP) {
                             // reserve one range per candidate, then place; a
                             // reservation past the candidate's row end is an
                             // error and its hits are dropped
@@ -1685,7 +1764,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     if (recording) {
         const uint64_t dirCap = std::max<uint64_t>(ctx->recDirCap, std::max<uint64_t>(1ull << 20, n));
         const uint64_t maskCap = std::max<uint64_t>(ctx->recMaskCap, std::max<uint64_t>(1ull << 23, 12 * n));
-        if (tBound > 0xFFFFFFFFull || recDir.alloc(pool, 2 * dirCap) != SJ_OK ||
+        if (tBound > 0xFFFFFFFFull || recDir.alloc(pool, (uint64_t)REC_DW * dirCap) != SJ_OK ||
             recMask.alloc(pool, maskCap) != SJ_OK || tileDir.alloc(pool, tBound + 1) != SJ_OK ||
             tileBound.alloc(pool, tBound) != SJ_OK || recCtr.alloc(pool, REC_NCTR) != SJ_OK) {
             recording = false;  // no room: the fill evaluates distances again
@@ -1740,7 +1819,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             for (uint32_t sl = 0; sl < REC_NSLICE; sl++) want += hc[REC_SLICE0 + sl * REC_STRIDE + 1];
             const uint64_t wantMask = std::max<uint64_t>(want + want / 4 + (uint64_t)REC_NSLICE * REC_BLK * 4,
                                                          pin[18] > a.dirCap ? a.maskCap : 2 * a.maskCap);
-            if ((wantDir * 16 + wantMask * 8) <= freeB / 4) {
+            if ((wantDir * 8 * REC_DW + wantMask * 8) <= freeB / 4) {
                 ctx->recDirCap = std::max(ctx->recDirCap, wantDir);
                 ctx->recMaskCap = std::max(ctx->recMaskCap, wantMask);
             }

@@ -56,7 +56,7 @@ struct JoinArgs {
     // 64-candidate piece, a directory entry {lanes with hits, first mask
     // word} and the hit masks of those lanes back to back; the fill replays
     // them instead of evaluating distances again.  recDir null: no record.
-    uint64_t *recDir;              // 2 words per entry
+    uint64_t *recDir;              // REC_DW words per entry
     uint64_t *recMask;
     const uint64_t *tileDir;       // tile t: entries [tileDir[t], tileDir[t+1])
     unsigned long long *recCtr;    // REC_* counters
@@ -69,6 +69,9 @@ struct JoinArgs {
 // REC_NSLICE); block b of slice c is global block b * REC_NSLICE + c.
 enum { REC_BAD = 0, REC_SLICE0 = 32 };
 constexpr uint32_t REC_BLK = 512;
+// Directory entry: {lanes with hits, first mask word, 1 + first word of the
+// chunk's mirror reservations (first piece of a chunk only; 0: none), 0}
+constexpr uint32_t REC_DW = 4;
 constexpr uint32_t REC_NSLICE = 64;
 constexpr uint32_t REC_STRIDE = 32;
 constexpr uint32_t REC_NCTR = REC_SLICE0 + REC_NSLICE * REC_STRIDE;
