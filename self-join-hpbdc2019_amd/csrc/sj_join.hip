@@ -617,10 +617,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                                 }
                             }
                         }
-                        // emission, one lane at a time: the lane's hit mask is
-                        // broadcast and each thread takes one candidate bit, so
-                        // the lane's hits leave as one contiguous row write;
-                        // mirrored hits bump the candidate's LDS counter
+                        // emission: mirrored hits (claimed non-own cells inside
+                        // the rows written) and own hits of the piece
                         uint64_t mirBits = 0ull;
                         if (anyMirror) {
                             uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
@@ -636,27 +634,64 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                             else err++;
                         }
                         const uint64_t wcur = ownPos + own;
-                        uint64_t todo = __ballot(ownBits != 0ull || mirBits != 0ull);
-                        // four source lanes per round so that their LDS reads overlap
-                        while (todo) {
-                            int l[4];
-                            bool lv[4];
-#pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                lv[u] = todo != 0ull;
-                                l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
-                                todo &= todo - 1ull;
+                        const uint32_t jb = pstart - c0;  // LDS row of the piece start (lanes with bits)
+                        // mirrored hits, lane by lane: the count and the
+                        // evaluating fill tally them per candidate; the
+                        // replayed fill places them in the recorded ranges
+                        if (anyMirror) {
+                            uint64_t mb = mirBits;
+                            while (mb) {
+                                const int bi = __builtin_ctzll(mb);
+                                mb &= mb - 1ull;
+                                const uint32_t jj = min(jb + (uint32_t)bi, (uint32_t)(CH - 1));
+                                if constexpr (RP) {
+                                    const uint32_t k2 = atomicAdd(&L.cnt[jj], 1u);
+                                    const unsigned long long bs = L.base[jj];
+                                    if (bs != NO_SLOT && k2 < (uint32_t)(bs & 127ull)) a.ids[(bs >> 7) + k2] = myId;
+                                    else if (bs != NO_SLOT) err++;
+                                } else {
+                                    atomicAdd(&L.cnt[jj], 1u);
+                                }
                             }
-                            uint64_t oB[4];
-                            uint32_t j[4];
+                        }
+                        // own hits: lane by lane when every lane has few, else
+                        // one source lane at a time, its mask broadcast so that
+                        // each thread takes one candidate bit and the hits
+                        // leave as one contiguous row write
+                        uint32_t mp = PASS == PASS_FILL ? (uint32_t)__popcll(ownBits) : 0u;
+                        if (PASS == PASS_FILL) {
 #pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                oB[u] = lv[u] ? readlane_u64(ownBits, l[u]) : 0ull;
-                                j[u] = min(readlane_u32(pstart, l[u]) + lane - c0, (uint32_t)(CH - 1));
+                            for (int off = 32; off > 0; off >>= 1) mp = max(mp, (uint32_t)__shfl_xor(mp, off, 64));
+                        }
+                        if (PASS == PASS_FILL && mp <= 8u) {
+                            uint64_t ob = ownBits;
+                            uint64_t wp = wcur;
+                            while (ob) {
+                                const int bi = __builtin_ctzll(ob);
+                                ob &= ob - 1ull;
+                                a.ids[wp++] = L.id[min(jb + (uint32_t)bi, (uint32_t)(CH - 1))];
                             }
-                            if (PASS == PASS_FILL) {
+                        } else if (PASS == PASS_FILL) {
+                            uint64_t todo = __ballot(ownBits != 0ull);
+                            // four source lanes per round so that their LDS reads overlap
+                            while (todo) {
+                                int l[4];
+                                bool lv[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    lv[u] = todo != 0ull;
+                                    l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+                                    todo &= todo - 1ull;
+                                }
+                                uint64_t oB[4];
+                                uint32_t j[4];
                                 int32_t idv[4];
                                 uint64_t w[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    oB[u] = lv[u] ? readlane_u64(ownBits, l[u]) : 0ull;
+                                    j[u] = min(readlane_u32(pstart, l[u]) + lane - c0, (uint32_t)(CH - 1));
+                                }
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
                                     idv[u] = L.id[j[u]];
@@ -665,24 +700,6 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
 #pragma unroll
                                 for (int u = 0; u < 4; u++)
                                     if ((oB[u] >> lane) & 1ull) a.ids[w[u]] = idv[u];
-                            }
-                            if (anyMirror) {
-#pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    const uint64_t mB = lv[u] ? readlane_u64(mirBits, l[u]) : 0ull;
-                                    if constexpr (RP) {
-                                        // placed now: slot k2 of the recorded range
-                                        const int32_t sid = (int32_t)readlane_u32((uint32_t)myId, l[u]);
-                                        if ((mB >> lane) & 1ull) {
-                                            const uint32_t k2 = atomicAdd(&L.cnt[j[u]], 1u);
-                                            const unsigned long long bs = L.base[j[u]];
-                                            if (bs != NO_SLOT && k2 < (uint32_t)(bs & 127ull)) a.ids[(bs >> 7) + k2] = sid;
-                                            else if (bs != NO_SLOT) err++;
-                                        }
-                                    } else {
-                                        if ((mB >> lane) & 1ull) atomicAdd(&L.cnt[j[u]], 1u);
-                                    }
-                                }
                             }
                         }
                         if (ownW) own += myCnt;
