@@ -400,9 +400,13 @@ def probe(a, torch):
     x = ctx.to_device(pts)
     form = sjamd.FORMS[a.form or spec["form"]]
     sched = sjamd.WORKQUEUE if (a.schedule or spec["schedule"]) == "workqueue" else sjamd.STRIDED
-    for prec in (sjamd.FP32_RECHECK, sjamd.FP64):
+    # the first join sizes the context's hit-record space; the measured
+    # joins then replay their records as in the timed steps
+    for prec in (sjamd.FP32_RECHECK, sjamd.FP32_RECHECK, sjamd.FP64):
         r = ctx.self_join(x, k, eps, form=form, schedule=sched, precision=prec)
-        print(json.dumps({"precision": prec, "pairs": r.total, "ms_fill": r.stats()["ms_fill"]}), flush=True)
+        st = r.stats()
+        print(json.dumps({"precision": prec, "pairs": r.total, "ms_fill": st["ms_fill"],
+                          "fill_from_record": st["fill_from_record"]}), flush=True)
         r.free()
 
 
@@ -429,7 +433,10 @@ def traffic_from(a):
         copy = max(rows[:first_sj], key=lambda r: r[2])
         factor = (COPY_BYTES / 1024.0) / copy[2]
         joins = [r for r in rows if "k_join" in r[1]]
-        kern = joins[1]  # probe order: FP32+recheck count, fill, then FP64 count, fill
+        # probe order: FP32+recheck count and fill twice, then FP64; the fill
+        # measured is the first FP32 one that replays the count's record
+        rp = [r for r in joins if "k_join_rp" in r[1] and r[1].rstrip(")").split("(")[0].endswith("1>")]
+        kern = rp[0] if rp else joins[3]
         out[counter] = {"copy_kernel": copy[1], "copy_kib": copy[2], "scale": factor,
                         "kernel": kern[1], "kib": kern[2], "bytes_scaled": kern[2] * 1024.0 * factor}
     res = {"workload": a.workload, "kernel": out["FETCH_SIZE"]["kernel"],

@@ -1781,7 +1781,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     const uint64_t tBound = (n + 63) / 64 + std::min<uint64_t>(n, g->nNonEmpty);
     if (recording) {
         const uint64_t dirCap = std::max<uint64_t>(ctx->recDirCap, std::max<uint64_t>(1ull << 20, n));
-        const uint64_t maskCap = std::max<uint64_t>(ctx->recMaskCap, std::max<uint64_t>(1ull << 23, 12 * n));
+        const uint64_t maskCap = std::max<uint64_t>(ctx->recMaskCap, std::max<uint64_t>(1ull << 23, 16 * n));
         if (tBound > 0xFFFFFFFFull || recDir.alloc(pool, (uint64_t)REC_DW * dirCap) != SJ_OK ||
             recMask.alloc(pool, maskCap) != SJ_OK || tileDir.alloc(pool, tBound + 1) != SJ_OK ||
             tileBound.alloc(pool, tBound) != SJ_OK || recCtr.alloc(pool, REC_NCTR) != SJ_OK) {
