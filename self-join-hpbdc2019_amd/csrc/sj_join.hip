@@ -233,11 +233,12 @@ struct TileLdsIds {
 };
 
 // Upper bound of the pieces a tile walk executes: per neighbour row in the
-// grid, ceil(staged candidates / 64), since chunks are whole multiples of
-// 64 candidates and a chunk runs at most ceil(chunk / 64) pieces.
+// grid, ceil(staged candidates / cw).  LDS tile walk: cw = 64 (chunks are
+// whole multiples of 64 candidates, a chunk runs at most ceil(chunk / 64)
+// pieces); matrix-core walk: cw = 32, one piece per 32-candidate chunk.
 template <int K>
 __device__ __forceinline__ uint32_t tile_piece_bound(const JoinArgs &a, const uint32_t *c, uint64_t rowBase,
-                                                     uint32_t xa, uint32_t xb) {
+                                                     uint32_t xa, uint32_t xb, uint32_t cw) {
     const uint32_t nC0 = a.nCells[0];
     const uint32_t ux0 = xa >= 1 ? xa - 1 : 0u;
     const uint32_t ux1 = xb + 1 < nC0 ? xb + 1 : nC0 - 1;
@@ -259,7 +260,7 @@ __device__ __forceinline__ uint32_t tile_piece_bound(const JoinArgs &a, const ui
         }
         if (inb) {
             const uint64_t rb = (uint64_t)((int64_t)rowBase + dl);
-            bound += (pos_lb(a, rb + ux1 + 1) - pos_lb(a, rb + ux0) + 63u) / 64u;
+            bound += (pos_lb(a, rb + ux1 + 1) - pos_lb(a, rb + ux0) + cw - 1u) / cw;
         }
         bool carry = true;
 #pragma unroll
@@ -581,42 +582,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                             }
                         }
                         }  // distance evaluation
-                        if (PASS == PASS_COUNT && recOn) {
-                            // record the piece: its directory entry, then the
-                            // masks of the lanes with hits, back to back in the
-                            // wave's current block of mask words
-                            const uint64_t pres = __ballot(mask != 0ull);
-                            const uint32_t nz = (uint32_t)__popcll(pres);
-                            R.want += nz;
-                            if (rec && dbase + piece_it >= dend) {
-                                rec = false;
-                                if (lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
-                            }
-                            if (rec && R.ok) {
-                                if (R.used + nz > REC_BLK) {
-                                    // next block from this wave's counter slice
-                                    const uint32_t sl = (blockIdx.x * WPB + (threadIdx.x >> 6)) % REC_NSLICE;
-                                    uint64_t b = 0;
-                                    if (lane == 0) b = atomicAdd(&a.recCtr[REC_SLICE0 + sl * REC_STRIDE], 1ull);
-                                    R.blk = (readlane_u64(b, 0) * REC_NSLICE + sl) * REC_BLK;
-                                    R.used = 0;
-                                    R.ok = R.blk + REC_BLK <= a.maskCap;
-                                    if (!R.ok && lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
-                                }
-                                if (R.ok) {
-                                    const uint64_t off = R.blk + R.used;
-                                    if (mask) a.recMask[off + popc_below64(pres)] = mask;
-                                    R.used += nz;
-                                    if (lane == 0) {
-                                        uint64_t *e = a.recDir + REC_DW * (dbase + piece_it);
-                                        e[0] = pres;
-                                        e[1] = off;
-                                        e[2] = 0ull;
-                                        e[3] = 0ull;
-                                    }
-                                }
-                            }
-                        }
+                        if (PASS == PASS_COUNT && recOn) record_piece(a, R, rec, dbase, dend, piece_it, mask);
                         // emission: mirrored hits (claimed non-own cells inside
                         // the rows written) and own hits of the piece
                         uint64_t mirBits = 0ull;
@@ -729,17 +695,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                                 rv[w] = ((uint64_t)b << 8) | v;
                                 nzm += (uint32_t)__popcll(hv[w]);
                             }
-                            if (recOn && rec && R.ok && np > 0 && nzm > (uint32_t)Cfg::MW) {
-                                if (R.used + nzm > REC_BLK) {
-                                    const uint32_t sl = (blockIdx.x * WPB + (threadIdx.x >> 6)) % REC_NSLICE;
-                                    uint64_t b = 0;
-                                    if (lane == 0) b = atomicAdd(&a.recCtr[REC_SLICE0 + sl * REC_STRIDE], 1ull);
-                                    R.blk = (readlane_u64(b, 0) * REC_NSLICE + sl) * REC_BLK;
-                                    R.used = 0;
-                                    R.ok = R.blk + REC_BLK <= a.maskCap;
-                                    if (!R.ok && lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
-                                }
-                                if (R.ok) {
+                            if (recOn && rec && np > 0 && nzm > (uint32_t)Cfg::MW) {
+                                if (rec_room(a, R, nzm)) {
                                     const uint64_t off = R.blk + R.used;
                                     uint32_t pre = Cfg::MW;
 #pragma unroll
@@ -958,7 +915,8 @@ __device__ __forceinline__ void many_mfma(const JoinArgs &a, uint32_t tile, unsi
         mfma_lds_clear(L);
         cleared = key;
     }
-    process_tile_mfma<DP, PASS>(a, tile, L, st);
+    TileRec R;  // many-join instances keep no hit record
+    process_tile_mfma<DP, PASS>(a, tile, L, st, R);
 }
 
 template <int PASS, int MF>
@@ -1176,10 +1134,10 @@ __global__ __launch_bounds__(JB) void k_tile_cost(JoinArgs a, uint32_t bound, ui
     val[t] = t;
 }
 
-// Pieces the LDS tile walk may execute per tile (tile_piece_bound); zero
+// Pieces a tile walk may execute per tile (tile_piece_bound); zero
 // for entries from nTiles to bound.  Scanned, they lay out the hit record.
 template <int K>
-__global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, uint32_t *out) {
+__global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, uint32_t cw, uint32_t *out) {
     const uint32_t t = blockIdx.x * JB + threadIdx.x;
     if (t >= bound) return;
     uint32_t v = 0;
@@ -1189,7 +1147,7 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 #pragma unroll
         for (int j = 0; j < K; j++) c[j] = a.cellCoord[(uint64_t)cA * K + j];
         const uint64_t rowBase = a.cellLinear[cA] - c[0];
-        v = tile_piece_bound<K>(a, c, rowBase, c[0], a.cellCoord[(uint64_t)cB * K]);
+        v = tile_piece_bound<K>(a, c, rowBase, c[0], a.cellCoord[(uint64_t)cB * K], cw);
     }
     out[t] = v;
 }
@@ -1356,7 +1314,7 @@ static TileCostKernel pick_tile_cost(int k) {
     return nullptr;
 }
 
-using TileBoundKernel = void (*)(JoinArgs, uint32_t, uint32_t *);
+using TileBoundKernel = void (*)(JoinArgs, uint32_t, uint32_t, uint32_t *);
 static TileBoundKernel pick_tile_bound(int k) {
     switch (k) {
     case 1: return k_tile_bound<1>;
@@ -1554,16 +1512,16 @@ static JoinKernel pick_replay(int k, int prec) {
     return nullptr;
 }
 
-// The count pass of these grids runs the LDS tile walk, which keeps a hit
-// record the fill can replay.
+// Walks whose count pass keeps a hit record the fill can replay: the LDS
+// tile walk (d = k <= 8) and the matrix-core walk (FP32, mDP > 0).
 static bool record_walk(const sj_grid_s *g, const JoinArgs &a) {
-    return a.useTiles && g->d == g->k && g->d <= DEV_MAXK;
+    return a.useTiles && ((g->d == g->k && g->d <= DEV_MAXK) || a.mDP > 0);
 }
 
 // replay: the fill replays the count pass's hit record (record_walk grids)
 static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, bool replay = false) {
     hipStream_t s = g->ctx->stream;
-    if (prec == SJ_PREC_FP32_RECHECK && a.mDP > 0) return launch_join_mfma(a, pass, g->ctx->cu_count, s);
+    if (prec == SJ_PREC_FP32_RECHECK && a.mDP > 0) return launch_join_mfma(a, pass, g->ctx->cu_count, s, replay);
     if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
         return launch_join_wide(a, pass, g->ctx->cu_count, s);
     JoinKernel kfn = replay ? pick_replay(g->k, prec) : pick_kernel(g->k, g->d, pass, prec);
@@ -1789,7 +1747,8 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             (void)hipGetLastError();
         } else {
             SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(recCtr.p), REC_NCTR, 0ull));
-            hipLaunchKernelGGL(pick_tile_bound(g->k), dim3(nblk(tBound)), dim3(JB), 0, s, a, (uint32_t)tBound,
+            const uint32_t cw = a.mDP > 0 ? 32u : 64u;  // candidates per recorded piece
+            hipLaunchKernelGGL(pick_tile_bound(g->k), dim3(nblk(tBound)), dim3(JB), 0, s, a, (uint32_t)tBound, cw,
                                tileBound.p);
             SJ_LAUNCH_CHECK();
             SJ_TRY(scan_exclusive_u32_to_u64(pool, s, tileBound.p, tBound, tileDir.p));

@@ -7,33 +7,49 @@
 
 namespace sj {
 
-template <int DP, int PASS>
-__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_mfma(JoinArgs a) {
-    __shared__ MfmaLds<DP> lds[WPB];
-    const unsigned wave = threadIdx.x >> 6;
+template <int DP, int PASS, bool RP>
+__device__ __forceinline__ void join_body_mfma(const JoinArgs &a, MfmaLds<DP> &L) {
     const unsigned lane = __lane_id();
-    MfmaLds<DP> &L = lds[wave];
     mfma_lds_clear(L);
     const uint32_t nTiles = *a.nTilesDev;
     LaneStats st;
+    TileRec R;
     if (a.tileOrder) {
         const uint32_t ns = min(gridDim.x, (uint32_t)QSLICES);
         const uint32_t sl = blockIdx.x % ns;
         const uint32_t nbl = (gridDim.x - sl + ns - 1) / ns;
-        uint64_t i = (uint64_t)(blockIdx.x / ns) * WPB + wave;
+        uint64_t i = (uint64_t)(blockIdx.x / ns) * WPB + (threadIdx.x >> 6);
         for (;;) {
             const uint64_t q = queue_entry(i, sl, ns);
             if (q >= nTiles) break;
             uint32_t nx = 0;
             if (lane == 0) nx = atomicAdd(a.qctr + sl * QSTRIDE, 1u);
-            process_tile_mfma<DP, PASS>(a, a.tileOrder[q], L, st);
+            process_tile_mfma<DP, PASS, RP>(a, a.tileOrder[q], L, st, R);
             i = (uint64_t)nbl * WPB + readlane_u32(nx, 0);
         }
     } else {
-        for (uint32_t tile = blockIdx.x * WPB + wave; tile < nTiles; tile += gridDim.x * WPB)
-            process_tile_mfma<DP, PASS>(a, tile, L, st);
+        for (uint32_t tile = blockIdx.x * WPB + (threadIdx.x >> 6); tile < nTiles; tile += gridDim.x * WPB)
+            process_tile_mfma<DP, PASS, RP>(a, tile, L, st, R);
     }
     flush_stats(a, st);
+    // mask words this wave's chunks needed, plus its last partial block
+    if (PASS == PASS_COUNT && a.recDir && lane == 0) {
+        const uint32_t sl = (blockIdx.x * WPB + (threadIdx.x >> 6)) % REC_NSLICE;
+        atomicAdd(&a.recCtr[REC_SLICE0 + sl * REC_STRIDE + 1], (unsigned long long)(R.want + REC_BLK));
+    }
+}
+
+template <int DP, int PASS>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_mfma(JoinArgs a) {
+    __shared__ MfmaLds<DP> lds[WPB];
+    join_body_mfma<DP, PASS, false>(a, lds[threadIdx.x >> 6]);
+}
+
+// Fill replaying the count pass's hit record
+template <int DP>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_mfma_rp(JoinArgs a) {
+    __shared__ MfmaLds<DP> lds[WPB];
+    join_body_mfma<DP, PASS_FILL, true>(a, lds[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------- host
@@ -96,7 +112,17 @@ void mfma_consts(JoinArgs &a, double T, double R) {
 }
 
 using MfmaKernel = void (*)(JoinArgs);
-static MfmaKernel pick_mfma(int DP, int pass) {
+static MfmaKernel pick_mfma(int DP, int pass, bool replay) {
+    if (replay) {
+        switch (DP) {
+        case 4: return k_join_mfma_rp<4>;
+        case 8: return k_join_mfma_rp<8>;
+        case 16: return k_join_mfma_rp<16>;
+        case 32: return k_join_mfma_rp<32>;
+        case 64: return k_join_mfma_rp<64>;
+        }
+        return nullptr;
+    }
     switch (DP) {
     case 4: return pass == PASS_COUNT ? k_join_mfma<4, PASS_COUNT> : k_join_mfma<4, PASS_FILL>;
     case 8: return pass == PASS_COUNT ? k_join_mfma<8, PASS_COUNT> : k_join_mfma<8, PASS_FILL>;
@@ -107,8 +133,8 @@ static MfmaKernel pick_mfma(int DP, int pass) {
     return nullptr;
 }
 
-sj_status launch_join_mfma(const JoinArgs &a, int pass, int cu_count, hipStream_t s) {
-    MfmaKernel kfn = pick_mfma(a.mDP, pass);
+sj_status launch_join_mfma(const JoinArgs &a, int pass, int cu_count, hipStream_t s, bool replay) {
+    MfmaKernel kfn = pick_mfma(a.mDP, pass, replay);
     if (!kfn) {
         set_error("no matrix-core kernel for padded dims %d", a.mDP);
         return SJ_ERR_UNSUPPORTED;

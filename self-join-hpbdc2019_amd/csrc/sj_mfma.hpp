@@ -46,9 +46,13 @@ struct MfmaLds {
     float nq[64];         // squared norms of the tile's queries
 };
 
-template <int DP, int PASS>
+// RP (fill only): take each chunk's hit masks from the count pass's hit
+// record (one directory entry per executed 32-candidate chunk) instead of
+// the matrix cores; chunks without hits are skipped.
+template <int DP, int PASS, bool RP = false>
 __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t tile, MfmaLds<DP> &L,
-                                                  LaneStats &st) {
+                                                  LaneStats &st, TileRec &R) {
+    static_assert(!RP || PASS == PASS_FILL, "only the fill replays a record");
     constexpr int HD = DP / 2;  // dims per lane half
     const unsigned lane = __lane_id();
     const unsigned jl = lane & 31u, g = lane >> 5;
@@ -77,7 +81,21 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
     const double *P = a.pts;
     const float *P32 = a.pts32;
     const bool src32 = a.mSrc32 != 0;  // stage from the FP32 copy (origin-relative bound)
+    // hit record: entries [tileDir[tile], tileDir[tile+1]), one per chunk
+    const bool recOn = a.recDir != nullptr;
+    bool rec = false;
+    uint64_t dbase = 0, dend = 0, piece_it = 0;
+    if (PASS == PASS_COUNT && recOn) {
+        dbase = readlane_u64(a.tileDir[tile], 0);
+        dend = readlane_u64(a.tileDir[tile + 1], 0);
+        rec = dend <= a.dirCap && !badTile;
+        if (!rec && lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
+    }
+    if constexpr (RP) dbase = readlane_u64(a.tileDir[tile], 0);
 
+    // operand A: queries 32h + jl, dims [g*HD, (g+1)*HD), centred in FP64
+    float A0[HD], A1[HD];
+    if constexpr (!RP) {
     // tile centre: the first point (the FP32 copy has rows of DP floats,
     // zero padded)
     if (src32) {
@@ -86,9 +104,6 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
         for (int j = (int)lane; j < d; j += 64) L.ctr[j] = P[(uint64_t)ts0 * d + j];
     }
     wave_lds_sync();
-
-    // operand A: queries 32h + jl, dims [g*HD, (g+1)*HD), centred in FP64
-    float A0[HD], A1[HD];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const uint64_t q = (uint64_t)ts0 + 32u * h + jl;
@@ -106,6 +121,7 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
         nq += __shfl_xor(nq, 32, 64);
         if (g == 0) L.nq[32 * h + jl] = nq;
     }
+    }  // operand A
 
     const bool ownW = valid && s >= a.wBegin && s < a.wEnd;
     const int32_t myId = valid ? (int32_t)a.origId[s] : 0;
@@ -174,6 +190,29 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
                 const bool mirC = anyMirror && !originRow;
                 for (uint32_t cb = u0; cb < u1; cb += 32) {
                     const uint32_t nv = min(32u, u1 - cb);
+                    if constexpr (RP) {
+                        const uint64_t *e = a.recDir + REC_DW * (dbase + piece_it);
+                        const uint64_t pres = e[0];
+                        piece_it++;
+                        if (!pres) continue;  // no hits in the chunk
+                        const uint32_t hit =
+                            (pres >> lane) & 1ull ? (uint32_t)a.recMask[e[1] + popc_below64(pres)] : 0u;
+                        if (lane < 32) {
+                            L.id[lane] = lane < nv ? (int32_t)a.origId[cb + lane] : -1;
+                            L.cnt[lane] = 0u;
+                        }
+                        wave_lds_sync();
+                        uint32_t mirBits = 0;
+                        if (anyMirror) {
+                            uint32_t elig = (uint32_t)range_bits(cb, e0, e1) | (uint32_t)range_bits(cb, e2, e3);
+                            if (mirC) elig |= (uint32_t)range_bits(cb, e1, e2);
+                            mirBits = hit & elig & (uint32_t)range_bits(cb, a.wBegin, a.wEnd);
+                        }
+                        emit_chunk32<PASS>(a, cb, nv, hit, mirBits, ownW, myId, ownPos, ownLim, own, L.id, L.cnt,
+                                           L.base, anyMirror, err);
+                        wave_lds_sync();
+                        continue;
+                    }
                     // stage: the chunk's rows are contiguous in the sorted copy
                     if (src32) {
                         // float4 f of the chunk goes to lane f % 64; the DP/4
@@ -280,6 +319,7 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
                             anyM = anyM || bm != 0ull;
                         }
                     }
+                    uint32_t hitRec = 0;  // the chunk's final hits, for the record
                     if (anyM) {
                         // claimed candidates of this lane's query in the chunk
                         uint32_t claim = 0;
@@ -313,7 +353,10 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
                         }
                         emit_chunk32<PASS>(a, cb, nv, hit, mirBits, ownW, myId, ownPos, ownLim, own, L.id, L.cnt,
                                            L.base, anyMirror, err);
+                        hitRec = hit;
                     }
+                    if (PASS == PASS_COUNT && recOn) record_piece(a, R, rec, dbase, dend, piece_it, (uint64_t)hitRec);
+                    piece_it++;
                     wave_lds_sync();
                 }
             }

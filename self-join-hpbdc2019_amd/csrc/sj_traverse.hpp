@@ -127,6 +127,49 @@ __device__ __forceinline__ double uniform_f64(double v) {
 
 constexpr unsigned long long NO_SLOT = ~0ull;
 
+// Room for nz more record words in the wave's current block, taking a new
+// block from the wave's counter slice when needed; false when the record
+// space is exhausted (the record is then unusable).
+__device__ __forceinline__ bool rec_room(const JoinArgs &a, TileRec &R, uint32_t nz) {
+    if (R.ok && R.used + nz > REC_BLK) {
+        const uint32_t sl = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % REC_NSLICE;
+        uint64_t b = 0;
+        if (__lane_id() == 0) b = atomicAdd(&a.recCtr[REC_SLICE0 + sl * REC_STRIDE], 1ull);
+        R.blk = (readlane_u64(b, 0) * REC_NSLICE + sl) * REC_BLK;
+        R.used = 0;
+        R.ok = R.blk + REC_BLK <= a.maskCap;
+        if (!R.ok && __lane_id() == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
+    }
+    return R.ok;
+}
+
+// Count pass, one executed piece it of a tile whose entries are
+// [dbase, dend): directory entry {lanes with hits, first mask word, 0, 0}
+// and the masks of the lanes with hits, back to back.
+__device__ __forceinline__ void record_piece(const JoinArgs &a, TileRec &R, bool &rec, uint64_t dbase,
+                                             uint64_t dend, uint64_t it, uint64_t mask) {
+    const unsigned lane = __lane_id();
+    const uint64_t pres = __ballot(mask != 0ull);
+    const uint32_t nz = (uint32_t)__popcll(pres);
+    R.want += nz;
+    if (rec && dbase + it >= dend) {
+        rec = false;
+        if (lane == 0) atomicAdd(&a.recCtr[REC_BAD], 1ull);
+    }
+    if (rec && rec_room(a, R, nz)) {
+        const uint64_t off = R.blk + R.used;
+        if (mask) a.recMask[off + popc_below64(pres)] = mask;
+        R.used += nz;
+        if (lane == 0) {
+            uint64_t *e = a.recDir + REC_DW * (dbase + it);
+            e[0] = pres;
+            e[1] = off;
+            e[2] = 0ull;
+            e[3] = 0ull;
+        }
+    }
+}
+
 // Entry i of queue slice sl of ns: slices take groups of QGROUP
 // consecutive entries in turn, so neighbouring tiles (in cell order inside
 // a cost bucket) stay on the XCD that serves the slice.
@@ -345,7 +388,8 @@ int mfma_padded_dims(int d);
 // R >= 0: staging from the FP32 copy whose coordinates are within R of the
 // origin (Euclidean), R < 0: staging centred in FP64.
 void mfma_consts(JoinArgs &a, double T, double R);
-sj_status launch_join_mfma(const JoinArgs &a, int pass, int cu_count, hipStream_t s);
+// replay: the fill replays the count pass's hit record
+sj_status launch_join_mfma(const JoinArgs &a, int pass, int cu_count, hipStream_t s, bool replay = false);
 
 // Own and mirrored hits of one staged chunk of at most 32 candidates at
 // sorted positions [cb, cb + nv).  hit: this lane's hits (bit c =
