@@ -896,17 +896,24 @@ struct LdsMax<0> {
 };
 constexpr int MANY_LDS = (LdsMax<DEV_MAXK>::v + 15) / 16 * 16;
 
-template <int K, int PASS, int PREC>
-__device__ __forceinline__ void many_tile(const JoinArgs &a, uint32_t tile, unsigned char *raw, LaneStats &st) {
-    TileRec R;  // many-join instances keep no hit record
-    if (PREC == 1) process_tile<K, K, PASS, 1, false>(a, tile, *reinterpret_cast<TileLds<K, 1> *>(raw), st, R);
-    else if (a.d == K) process_tile<K, K, PASS, 0, false>(a, tile, *reinterpret_cast<TileLds<K, 0> *>(raw), st, R);
-    else process_tile_wide<PASS>(a, tile, *reinterpret_cast<WideLds *>(raw), st);
+template <int K, int PASS, int PREC, bool RP>
+__device__ __forceinline__ void many_tile(const JoinArgs &a, uint32_t tile, unsigned char *raw, LaneStats &st,
+                                          TileRec &R) {
+    if constexpr (RP) {
+        // the LDS tile walks replay their record; the wide walk keeps none
+        if (PREC == 1) process_tile<K, K, PASS, 1, true>(a, tile, *reinterpret_cast<TileLdsIds<K, 1> *>(raw), st, R);
+        else if (a.d == K) process_tile<K, K, PASS, 0, true>(a, tile, *reinterpret_cast<TileLdsIds<K, 0> *>(raw), st, R);
+        else process_tile_wide<PASS>(a, tile, *reinterpret_cast<WideLds *>(raw), st);
+    } else {
+        if (PREC == 1) process_tile<K, K, PASS, 1, false>(a, tile, *reinterpret_cast<TileLds<K, 1> *>(raw), st, R);
+        else if (a.d == K) process_tile<K, K, PASS, 0, false>(a, tile, *reinterpret_cast<TileLds<K, 0> *>(raw), st, R);
+        else process_tile_wide<PASS>(a, tile, *reinterpret_cast<WideLds *>(raw), st);
+    }
 }
 
-template <int DP, int PASS>
+template <int DP, int PASS, bool RP>
 __device__ __forceinline__ void many_mfma(const JoinArgs &a, uint32_t tile, unsigned char *raw, int &cleared,
-                                          LaneStats &st) {
+                                          LaneStats &st, TileRec &R) {
     // the pad columns [d, DP) of the staged rows must be zero: clear when
     // (DP, d) changes or another walk used the LDS
     MfmaLds<DP> &L = *reinterpret_cast<MfmaLds<DP> *>(raw);
@@ -915,11 +922,13 @@ __device__ __forceinline__ void many_mfma(const JoinArgs &a, uint32_t tile, unsi
         mfma_lds_clear(L);
         cleared = key;
     }
-    TileRec R;  // many-join instances keep no hit record
-    process_tile_mfma<DP, PASS>(a, tile, L, st, R);
+    process_tile_mfma<DP, PASS, RP>(a, tile, L, st, R);
 }
 
-template <int PASS, int MF>
+// RP (fill): the instances' LDS tile and matrix-core walks replay the hit
+// record of the count kernel; a record marked unusable makes every item an
+// error instead.
+template <int PASS, int MF, bool RP>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_many(const JoinArgs *__restrict__ args,
                                                   const uint32_t *__restrict__ workInst,
                                                   const uint32_t *__restrict__ workTile, uint32_t W,
@@ -929,6 +938,7 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const unsigned lane = __lane_id();
     unsigned char *raw = lds[wave];
     LaneStats st;
+    TileRec R;  // the wave's block of record words, across its items
     uint32_t cur = 0xFFFFFFFFu;
     int cleared = 0;  // (DP, d) whose zero pad columns the LDS holds, 0: none
     auto run = [&](uint32_t w) __attribute__((always_inline)) {
@@ -940,26 +950,30 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
             cur = inst;
         }
         const JoinArgs &a = args[inst];
+        if (RP && a.recCtr[REC_BAD]) {
+            if (lane == 0) st.err++;
+            return;
+        }
         if (MF && a.mDP > 0) {
             switch (a.mDP) {
-            case 4: many_mfma<4, PASS>(a, tile, raw, cleared, st); break;
-            case 8: many_mfma<8, PASS>(a, tile, raw, cleared, st); break;
-            case 16: many_mfma<16, PASS>(a, tile, raw, cleared, st); break;
-            case 32: many_mfma<32, PASS>(a, tile, raw, cleared, st); break;
-            case 64: many_mfma<64, PASS>(a, tile, raw, cleared, st); break;
+            case 4: many_mfma<4, PASS, RP>(a, tile, raw, cleared, st, R); break;
+            case 8: many_mfma<8, PASS, RP>(a, tile, raw, cleared, st, R); break;
+            case 16: many_mfma<16, PASS, RP>(a, tile, raw, cleared, st, R); break;
+            case 32: many_mfma<32, PASS, RP>(a, tile, raw, cleared, st, R); break;
+            case 64: many_mfma<64, PASS, RP>(a, tile, raw, cleared, st, R); break;
             }
             return;
         }
         cleared = 0;
         switch (a.k) {
-        case 1: many_tile<1, PASS, MF>(a, tile, raw, st); break;
-        case 2: many_tile<2, PASS, MF>(a, tile, raw, st); break;
-        case 3: many_tile<3, PASS, MF>(a, tile, raw, st); break;
-        case 4: many_tile<4, PASS, MF>(a, tile, raw, st); break;
-        case 5: many_tile<5, PASS, MF>(a, tile, raw, st); break;
-        case 6: many_tile<6, PASS, MF>(a, tile, raw, st); break;
-        case 7: many_tile<7, PASS, MF>(a, tile, raw, st); break;
-        case 8: many_tile<8, PASS, MF>(a, tile, raw, st); break;
+        case 1: many_tile<1, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 2: many_tile<2, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 3: many_tile<3, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 4: many_tile<4, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 5: many_tile<5, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 6: many_tile<6, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 7: many_tile<7, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 8: many_tile<8, PASS, MF, RP>(a, tile, raw, st, R); break;
         }
     };
     if (useQueue) {
@@ -980,7 +994,14 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
     } else {
         for (uint32_t w = blockIdx.x * WPB + wave; w < W; w += gridDim.x * WPB) run(w);
     }
-    if (cur != 0xFFFFFFFFu) flush_stats(args[cur], st);
+    if (cur != 0xFFFFFFFFu) {
+        flush_stats(args[cur], st);
+        // record words this wave's items needed (the record is shared)
+        if (PASS == PASS_COUNT && args[cur].recDir && lane == 0) {
+            const uint32_t sl = (blockIdx.x * WPB + wave) % REC_NSLICE;
+            atomicAdd(&args[cur].recCtr[REC_SLICE0 + sl * REC_STRIDE + 1], (unsigned long long)(R.want + REC_BLK));
+        }
+    }
 }
 
 // Tiles: runs of at most 64 consecutive points of one cell row within
@@ -1541,15 +1562,29 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
 
 sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
                            uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, int mfma,
-                           unsigned blocks, hipStream_t s) {
+                           unsigned blocks, hipStream_t s, bool replay) {
     if (W == 0) return SJ_OK;
     if (chunk == 0) chunk = 1;
     if (blocks == 0) blocks = 1;
     using ManyKernel = void (*)(const JoinArgs *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, uint32_t,
                                 int);
-    const ManyKernel kfn = pass == PASS_COUNT ? (mfma ? k_join_many<PASS_COUNT, 1> : k_join_many<PASS_COUNT, 0>)
-                                              : (mfma ? k_join_many<PASS_FILL, 1> : k_join_many<PASS_FILL, 0>);
+    ManyKernel kfn;
+    if (pass == PASS_COUNT) kfn = mfma ? k_join_many<PASS_COUNT, 1, false> : k_join_many<PASS_COUNT, 0, false>;
+    else if (replay) kfn = mfma ? k_join_many<PASS_FILL, 1, true> : k_join_many<PASS_FILL, 0, true>;
+    else kfn = mfma ? k_join_many<PASS_FILL, 1, false> : k_join_many<PASS_FILL, 0, false>;
     hipLaunchKernelGGL(kfn, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W, qctr, chunk, useQueue);
+    SJ_LAUNCH_CHECK();
+    return SJ_OK;
+}
+
+sj_status launch_tile_bound(const JoinArgs &a, int k, uint32_t bound, uint32_t cw, uint32_t *out, hipStream_t s) {
+    TileBoundKernel kfn = pick_tile_bound(k);
+    if (!kfn) {
+        set_error("tile bound: k=%d above the device stencil limit", k);
+        return SJ_ERR_UNSUPPORTED;
+    }
+    if (bound == 0) return SJ_OK;
+    hipLaunchKernelGGL(kfn, dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, cw, out);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }

@@ -1,5 +1,6 @@
 // Many independent self-joins (different n, d, k and eps) in one
-// persistent launch per pass, recorded once into a hipGraph.
+// persistent launch per pass, recorded once into a hipGraph.  The count
+// kernel keeps a hit record that the fill kernel replays.
 //
 // The reference runs one join per process (main.cu:36-226); a batch of
 // small joins there is a sequence of under-filled launches.  Here a plan
@@ -101,6 +102,13 @@ struct sj_many_s {
     std::vector<unsigned long long> hStat;
     double ms_plan = 0.0, ms_grid = 0.0, ms_run = 0.0;
     int mfma = 0;  // 1: every instance runs the matrix-core FP32 walk
+    // hit record shared by the instances: the count kernel records, the fill
+    // kernel replays (the wide walk's instances evaluate again)
+    bool rec = false;
+    DBuf<uint32_t> tBound;
+    DBuf<uint64_t> tileDir, recDir, recMask;
+    DBuf<unsigned long long> recCtr;
+    uint64_t dirCap = 0, maskCap = 0;
 
     ~sj_many_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -133,6 +141,7 @@ sj_status enqueue_pipeline(sj_many_s *m, bool fill) {
     SJ_TRY(fill_u32(s, m->mir.p, N, 0u));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(m->stat.p), (uint64_t)cnt * 8, 0ull));
     SJ_TRY(fill_u32(s, m->qctr.p, 1, 0u));
+    if (m->rec) SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(m->recCtr.p), REC_NCTR, 0ull));
     SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_COUNT, m->mfma, blocks,
                            s));
     hipLaunchKernelGGL(k_many_totals, dim3(nb(N)), dim3(JB), 0, s, m->own.p, m->mir.p, N, m->tot.p);
@@ -143,7 +152,7 @@ sj_status enqueue_pipeline(sj_many_s *m, bool fill) {
     SJ_LAUNCH_CHECK();
     SJ_TRY(fill_u32(s, m->qctr.p, 1, 0u));
     SJ_TRY(launch_join_many(m->dargs.p, wi, wt, m->W, m->qctr.p, chunk, queue ? 1 : 0, PASS_FILL, m->mfma, blocks,
-                           s));
+                           s, m->rec));
     const uint64_t E = N + (uint64_t)cnt;
     hipLaunchKernelGGL(k_many_rebase, dim3(nb(E)), dim3(JB), 0, s, m->G.p, m->base.p, m->relBase.p, cnt, E,
                        m->rel.p, m->idsBase.p);
@@ -186,7 +195,33 @@ sj_status capture(sj_many_s *m) {
     return SJ_OK;
 }
 
-// One replay plus the checks: |S| unchanged, no inconsistent writes.
+// Mask words the record needed in the last count (the slices' demand).
+sj_status rec_want(sj_many_s *m, uint64_t *want, uint64_t *bad) {
+    std::vector<unsigned long long> hc(REC_NCTR);
+    SJ_TRY_HIP(hipMemcpy(hc.data(), m->recCtr.p, REC_NCTR * 8, hipMemcpyDeviceToHost));
+    uint64_t w = 0;
+    for (uint32_t sl = 0; sl < REC_NSLICE; sl++) w += hc[REC_SLICE0 + sl * REC_STRIDE + 1];
+    *want = w;
+    *bad = hc[REC_BAD];
+    return SJ_OK;
+}
+
+// Record space for `words` mask words (the slices may fill unevenly: the
+// callers ask for several times the demand seen).
+sj_status rec_space(sj_many_s *m, uint64_t words) {
+    if (words <= m->maskCap) return SJ_OK;
+    SJ_TRY(m->recMask.alloc(&m->ctx->pool, words));
+    m->maskCap = words;
+    for (auto &a : m->args) {
+        a.recMask = m->recMask.p;
+        a.maskCap = words;
+    }
+    return SJ_OK;
+}
+
+// One replay plus the checks: |S| unchanged, no inconsistent writes.  A
+// record that outgrew its space (every fill item then reports an error)
+// grows the space, the graph is recorded again and the replay repeated.
 sj_status run_once(sj_many_s *m, double *ms) {
     hipStream_t s = m->ctx->stream;
     hipEvent_t e0, e1;
@@ -216,6 +251,20 @@ sj_status run_once(sj_many_s *m, double *ms) {
         set_error("many-join: |S| changed between plan (%llu) and run (%llu)", (unsigned long long)m->total,
                   (unsigned long long)tot);
         return SJ_ERR_STATE;
+    }
+    if (m->rec) {
+        uint64_t want = 0, bad = 0;
+        SJ_TRY(rec_want(m, &want, &bad));
+        if (bad) {
+            if (m->maskCap >= (1ull << 36)) {
+                set_error("many-join: the hit record outgrew %llu words", (unsigned long long)m->maskCap);
+                return SJ_ERR_NOMEM;
+            }
+            SJ_TRY(rec_space(m, std::max<uint64_t>(2 * m->maskCap, 3 * want)));
+            SJ_TRY(upload_args(m));
+            SJ_TRY(capture(m));
+            return run_once(m, ms);
+        }
     }
     for (size_t i = 0; i < cnt; i++)
         if (m->hStat[i * 8 + STAT_ERR]) {
@@ -355,12 +404,50 @@ sj_status plan(sj_many_s *m, const sj_instance *in, int count) {
         SJ_TRY_HIP(hipStreamSynchronize(s));
     }
 
-    // size the shared ID array with one eager count run
+    // hit record: per-tile bounds of every instance, end to end, scanned into
+    // directory bases; one directory and word space for all instances
+    if (P.hit_record >= 0 && W) {
+        SJ_TRY(m->tBound.alloc(pool, W));
+        SJ_TRY(m->tileDir.alloc(pool, (uint64_t)W + 1));
+        uint64_t w0 = 0;
+        for (int i = 0; i < count; i++) {
+            const uint32_t cw = m->args[i].mDP > 0 ? 32u : 64u;
+            SJ_TRY(launch_tile_bound(m->args[i], m->args[i].k, m->inst[i].nTiles, cw, m->tBound.p + w0, s));
+            w0 += m->inst[i].nTiles;
+        }
+        SJ_TRY(scan_exclusive_u32_to_u64(pool, s, m->tBound.p, W, m->tileDir.p));
+        SJ_TRY_HIP(hipMemcpyAsync(&m->dirCap, m->tileDir.p + W, 8, hipMemcpyDeviceToHost, s));
+        SJ_TRY_HIP(hipStreamSynchronize(s));
+        m->dirCap = std::max<uint64_t>(m->dirCap, 1);
+        SJ_TRY(m->recDir.alloc(pool, (uint64_t)REC_DW * m->dirCap));
+        SJ_TRY(m->recCtr.alloc(pool, REC_NCTR));
+        SJ_TRY(rec_space(m, std::max<uint64_t>(1ull << 23, 16 * N)));
+        w0 = 0;
+        for (int i = 0; i < count; i++) {
+            JoinArgs &a = m->args[i];
+            a.recDir = m->recDir.p;
+            a.recMask = m->recMask.p;
+            a.tileDir = m->tileDir.p + w0;
+            a.recCtr = m->recCtr.p;
+            a.dirCap = m->dirCap;
+            a.maskCap = m->maskCap;
+            w0 += m->inst[i].nTiles;
+        }
+        m->rec = true;
+    }
+
+    // size the shared ID array with one eager count run (which also sizes
+    // the hit record: three times the words it needed)
     SJ_TRY(m->dargs.alloc(pool, count));
     SJ_TRY(upload_args(m));
     SJ_TRY(enqueue_pipeline(m, false));
     SJ_TRY_HIP(hipStreamSynchronize(s));
     SJ_TRY_HIP(hipMemcpy(&m->total, m->G.p + N, 8, hipMemcpyDeviceToHost));
+    if (m->rec) {
+        uint64_t want = 0, bad = 0;
+        SJ_TRY(rec_want(m, &want, &bad));
+        SJ_TRY(rec_space(m, 3 * want));
+    }
     SJ_TRY(m->ids.alloc(pool, m->total));
     for (int i = 0; i < count; i++) m->args[i].ids = m->ids.p;
     SJ_TRY(upload_args(m));

@@ -485,9 +485,14 @@ sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec);
 
 // One persistent launch over (instance, tile) work items of many joins;
 // mfma: every instance runs the matrix-core FP32 walk, else the FP64 walks.
+// replay (fill): the instances' LDS tile and matrix-core walks replay the
+// hit record of the count kernel
 sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, const uint32_t *workTile,
                            uint32_t W, uint32_t *qctr, uint32_t chunk, int useQueue, int pass, int mfma,
-                           unsigned blocks, hipStream_t s);
+                           unsigned blocks, hipStream_t s, bool replay = false);
+// Per-tile bound of the pieces a walk executes (cw candidates per piece)
+// for the first `bound` tiles of a.
+sj_status launch_tile_bound(const JoinArgs &a, int k, uint32_t bound, uint32_t cw, uint32_t *out, hipStream_t s);
 
 // Work-queue keys of tiles [0, bound) of one join (descending 8-bit
 // log-scale bucket of the staged candidates; val[t] = t).
