@@ -108,3 +108,43 @@ def test_record_with_batches(ctx):
         _same(r.csr(), ref, f"batches {form}")
         r.free()
     g.free()
+
+
+MFMA_CASES = {
+    "clusters16d_k4_6k": lambda: (D.clusters(6_000, 16, 21), 4, 2.4),
+    "uniform20d_k3_4k": lambda: (D.uniform(4_000, 20, 22), 3, 1.05),
+}
+
+
+@pytest.mark.parametrize("name", list(MFMA_CASES))
+@pytest.mark.parametrize("form", ["full", "bunicomp", "lid"])
+def test_matrix_core_replay_matches(name, form):
+    """FP32+recheck on the matrix cores (d > 8): the second join on a fresh
+    context replays the count's chunk record; whole and batched results
+    equal the oracle"""
+    import sjamd
+
+    if name not in _cache:
+        pts, k, eps = MFMA_CASES[name]()
+        _cache[name] = (pts, k, eps, O.join(pts, k, eps, O.LID))
+    pts, k, eps, ref = _cache[name]
+    assert int(ref[0][-1]) > pts.shape[0]
+    _say(f"[record mfma] {name} {form}")
+    c = sjamd.Context(0)
+    try:
+        g = c.grid_build(pts, k, eps)
+        for sched in (sjamd.STRIDED, sjamd.WORKQUEUE):
+            r1 = g.join(form=form, precision=sjamd.FP32_RECHECK, schedule=sched)
+            _same(r1.csr(), ref, f"{name} {form} sched={sched} first")
+            r1.free()
+            r2 = g.join(form=form, precision=sjamd.FP32_RECHECK, schedule=sched)
+            assert r2.stats()["fill_from_record"] == 1
+            _same(r2.csr(), ref, f"{name} {form} sched={sched} replayed")
+            r2.free()
+        r3 = g.join(form=form, precision=sjamd.FP32_RECHECK, capacity=int(ref[0][-1]) // 4 + 1)
+        assert r3.stats()["num_batches"] >= 3 and r3.stats()["fill_from_record"] == 1
+        _same(r3.csr(), ref, f"{name} {form} batches")
+        r3.free()
+        g.free()
+    finally:
+        c.close()
