@@ -211,22 +211,58 @@ __global__ __launch_bounds__(GB) void k_cell_coords(const double *spts, const ui
 // posLB[L], L in [0, tc]: the first sorted position whose cell has linear
 // ID >= L, i.e. cellStart of the first non-empty cell at or after L (n past
 // the last).  A cell L is non-empty iff posLB[L] < posLB[L + 1].  Thread t
-// fills the run [t * POSLB_RUN, (t + 1) * POSLB_RUN): one binary search of
-// the sorted cell IDs, then a forward walk.
+// fills the run [t * POSLB_RUN, (t + 1) * POSLB_RUN).  The wave first finds
+// the first cell at or after its lowest ID with a 64-way search (each round
+// the 64 lanes probe evenly spaced cells); a lane's own first cell then lies
+// within (its ID - the wave's ID) cells of it, found by a short binary
+// search; a forward walk fills the run, stored 16 bytes at a time.
 constexpr uint32_t POSLB_RUN = 16;
 __global__ __launch_bounds__(GB) void k_pos_lb(const uint64_t *cellLinear, const uint32_t *cellStart,
                                                uint64_t nne, uint64_t tc, uint32_t *posLB) {
-    const uint64_t L0 = ((uint64_t)blockIdx.x * GB + threadIdx.x) * POSLB_RUN;
-    if (L0 > tc) return;
+    const uint64_t t = (uint64_t)blockIdx.x * GB + threadIdx.x;
+    const unsigned lane = __lane_id();
+    const uint64_t Lw = (t - lane) * POSLB_RUN;  // the wave's lowest ID
+    if (Lw > tc) return;                          // wave-uniform
+    // first index with cellLinear >= Lw, within [lo, hi]
     uint64_t lo = 0, hi = nne;
     while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (cellLinear[mid] < L0) lo = mid + 1; else hi = mid;
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t pos = lo + step * lane;
+        const bool less = pos < hi && cellLinear[pos] < Lw;
+        const uint32_t c = (uint32_t)__popcll(__ballot(less));  // probes below Lw: a prefix
+        if (c == 0) {
+            hi = lo;
+        } else {
+            const uint64_t last = lo + step * (c - 1);  // below Lw
+            const uint64_t next = lo + step * c;        // at or above Lw when < hi
+            lo = last + 1;
+            hi = next < hi ? next : hi;
+        }
+    }
+    const uint64_t L0 = t * POSLB_RUN;
+    if (L0 > tc) return;
+    // this lane's first cell: at most L0 - Lw cells past the wave's
+    uint64_t a = lo, b = lo + (L0 - Lw) < nne ? lo + (L0 - Lw) : nne;
+    while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (cellLinear[mid] < L0) a = mid + 1; else b = mid;
     }
     const uint64_t L1 = L0 + POSLB_RUN < tc + 1 ? L0 + POSLB_RUN : tc + 1;
-    for (uint64_t L = L0; L < L1; L++) {
-        while (lo < nne && cellLinear[lo] < L) lo++;
-        posLB[L] = cellStart[lo];
+    uint32_t v[POSLB_RUN];
+#pragma unroll
+    for (uint32_t q = 0; q < POSLB_RUN; q++) {
+        const uint64_t L = L0 + q;
+        while (a < nne && cellLinear[a] < L) a++;
+        v[q] = cellStart[a];
+    }
+    if (L1 - L0 == POSLB_RUN) {
+        uint4 *o = reinterpret_cast<uint4 *>(posLB + L0);
+#pragma unroll
+        for (uint32_t q = 0; q < POSLB_RUN / 4; q++) o[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < POSLB_RUN; q++)
+            if (L0 + q < L1) posLB[L0 + q] = v[q];
     }
 }
 
