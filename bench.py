@@ -457,7 +457,7 @@ def main():
         traffic_from(a)
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
               "(the join would shard by query cells over a replicated grid)", file=sys.stderr)
         sys.exit(2)
     import torch
