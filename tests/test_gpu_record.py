@@ -148,3 +148,21 @@ def test_matrix_core_replay_matches(name, form):
         g.free()
     finally:
         c.close()
+
+
+def test_batches_without_record(ctx):
+    """result rounds whose fill evaluates distances again over the count
+    pass's tiles (hit record off), every form and both precisions"""
+    import sjamd
+
+    pts, k, eps, ref = _case("uniform6d_50k")
+    _say("[record] batches without record uniform6d_50k")
+    g = ctx.grid_build(pts, k, eps)
+    for form in ("full", "unicomp", "bunicomp", "lid"):
+        for prec in (sjamd.FP64, sjamd.FP32_RECHECK):
+            r = g.join(form=form, precision=prec, capacity=int(ref[0][-1]) // 5 + 1, hit_record=False)
+            st = r.stats()
+            assert st["num_batches"] >= 4 and st["fill_from_record"] == 0
+            _same(r.csr(), ref, f"batches without record {form} prec={prec}")
+            r.free()
+    g.free()
