@@ -379,6 +379,9 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     }
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
     const bool anyMirror = a.form != SJ_FORM_FULL;
+    // a fill writes only rows [wBegin, wEnd): chunks without such candidates
+    // matter only to tiles that own rows there
+    const bool anyOwnW = __ballot(ownW) != 0ull;
 
     // neighbour rows: odometer over dims 1..K-1
     int o[K];
@@ -449,6 +452,10 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     for (int w = 0; w < Cfg::MW; w++) rm[w] = 0ull;
                     const uint32_t np = (maxLen + 63u) / 64u;  // pieces of the chunk
                     const uint64_t chunkIt = piece_it;
+                    if (PASS == PASS_FILL && !anyOwnW && (c0 + m <= a.wBegin || c0 >= a.wEnd)) {
+                        piece_it += np;  // nothing of this chunk lands in the rows written
+                        continue;
+                    }
                     if constexpr (RP) {
                         uint64_t anyHit = 0ull;
 #pragma unroll

@@ -133,6 +133,9 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
     }
     uint64_t cand = 0, band = 0, mis = 0, err = 0;
     const bool anyMirror = a.form != SJ_FORM_FULL;
+    // a fill writes only rows [wBegin, wEnd): chunks without such candidates
+    // matter only to tiles that own rows there
+    const bool anyOwnW = __ballot(ownW) != 0ull;
     // staging: element t of a chunk (row t / d, dim t % d) goes to lane t % 64
     const uint32_t cstep = 64u / (uint32_t)d, jstep = 64u % (uint32_t)d;
     const uint32_t c0l = lane / (uint32_t)d, j0l = lane % (uint32_t)d;
@@ -190,6 +193,10 @@ __device__ __forceinline__ void process_tile_mfma(const JoinArgs &a, uint32_t ti
                 const bool mirC = anyMirror && !originRow;
                 for (uint32_t cb = u0; cb < u1; cb += 32) {
                     const uint32_t nv = min(32u, u1 - cb);
+                    if (PASS == PASS_FILL && !anyOwnW && (cb + nv <= a.wBegin || cb >= a.wEnd)) {
+                        piece_it++;  // nothing of this chunk lands in the rows written
+                        continue;
+                    }
                     if constexpr (RP) {
                         const uint64_t *e = a.recDir + REC_DW * (dbase + piece_it);
                         const uint64_t pres = e[0];
