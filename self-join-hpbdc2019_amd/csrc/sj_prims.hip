@@ -235,32 +235,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const Tin *in, uint64
     if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
-template <class Tin, class Tacc>
-static sj_status scan_exclusive(Pool *pool, hipStream_t s, const Tin *in, uint64_t n, Tacc *out) {
-    if (n == 0) {
-        hipLaunchKernelGGL(k_fill<Tacc>, dim3(1), dim3(64), 0, s, out, (uint64_t)1, (Tacc)0);
-        SJ_LAUNCH_CHECK();
-        return SJ_OK;
-    }
-    uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    DBuf<Tacc> sums;
-    SJ_TRY(sums.alloc(pool, nb + 1));
-    hipLaunchKernelGGL((k_scan_reduce<Tin, Tacc>), dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n,
-                       sums.p);
-    SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scan_sums<Tacc>), dim3(1), dim3(SCAN_BLOCK), 0, s, sums.p, nb);
-    SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scan_apply<Tin, Tacc>), dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n,
-                       sums.p, out, nb);
-    SJ_LAUNCH_CHECK();
-    return SJ_OK;
-}
-
-sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
-                                    uint64_t *out) {
-    return scan_exclusive<uint32_t, uint64_t>(pool, s, in, n, out);
-}
-
 uint64_t scan_temp_count(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 // Same scan with a caller-owned block-sum buffer of scan_temp_count(n)
@@ -282,11 +256,6 @@ sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint6
                        sums, out, nb);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
-}
-
-sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
-                                    uint32_t *out) {
-    return scan_exclusive<uint32_t, uint32_t>(pool, s, in, n, out);
 }
 
 // -------------------------------------------------------------- radix sort
