@@ -264,7 +264,8 @@ sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint6
 // digit-major table gives each (digit, tile) its output base; the
 // downsweep ranks each round of 256 keys stably with wave64 ballots
 // (peer mask of equal digits, popcount below the lane) plus a prefix over
-// the four waves, and scatters.
+// the four waves, places the tile's keys in LDS in digit order and writes
+// them out run by run.
 constexpr int RS_BLOCK = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr uint64_t RS_TILE = (uint64_t)RS_BLOCK * RS_ROUNDS;
@@ -292,12 +293,21 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m) {
 template <class K>
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K *kin, const uint32_t *vin, K *kout,
                                                             uint32_t *vout, uint64_t n, int shift,
-                                                            const uint32_t *histScan,
+                                                            const uint32_t *histScan, const uint32_t *hist,
                                                             uint32_t ntiles) {
-    __shared__ uint32_t base[256];
+    __shared__ uint32_t base[256];   // next output position of each digit
+    __shared__ uint32_t gbase[256];  // the tile's first output position of each digit
+    __shared__ uint32_t lbase[256];  // the tile's first slot of each digit in digit order
     __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+    __shared__ K sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
     const unsigned wave = threadIdx.x >> 6;
-    base[threadIdx.x] = histScan[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    const uint32_t gb = histScan[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    base[threadIdx.x] = gb;
+    gbase[threadIdx.x] = gb;
+    uint32_t tileCount;
+    lbase[threadIdx.x] =
+        block_exclusive_scan<uint32_t>(hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x], &tileCount);
     const uint64_t tbase = (uint64_t)blockIdx.x * RS_TILE;
     for (int j = 0; j < RS_ROUNDS; j++) {
         const uint64_t i = tbase + (uint64_t)j * RS_BLOCK + threadIdx.x;
@@ -330,11 +340,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K *kin, const 
         }
         __syncthreads();
         if (valid) {
-            uint32_t pos = wcnt[wave][dg] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+            // the key's output position, and its slot in the tile's digit order
+            const uint32_t pos = wcnt[wave][dg] + rank;
+            const uint32_t lp = lbase[dg] + (pos - gbase[dg]);
+            sk[lp] = key;
+            sv[lp] = val;
         }
         __syncthreads();
+    }
+    // the tile in digit order: a run of equal digits goes to consecutive
+    // output positions, so the writes coalesce
+    const uint32_t cnt = (uint32_t)min<uint64_t>(RS_TILE, n - tbase);
+    for (uint32_t i = threadIdx.x; i < cnt; i += RS_BLOCK) {
+        const K key = sk[i];
+        const uint32_t dg = (uint32_t)(key >> shift) & 255u;
+        const uint32_t pos = gbase[dg] + (i - lbase[dg]);
+        kout[pos] = key;
+        vout[pos] = sv[i];
     }
 }
 
@@ -356,7 +378,7 @@ static sj_status radix_sort_pairs(Pool *pool, hipStream_t s, K **keys, K **keys_
         SJ_LAUNCH_CHECK();
         SJ_TRY(scan_exclusive_u32_to_u32(pool, s, hist.p, (uint64_t)256 * ntiles, histScan.p));
         hipLaunchKernelGGL((k_radix_scatter<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, *vals,
-                           *keys_alt, *vals_alt, n, shift, histScan.p, ntiles);
+                           *keys_alt, *vals_alt, n, shift, histScan.p, hist.p, ntiles);
         SJ_LAUNCH_CHECK();
         std::swap(*keys, *keys_alt);
         std::swap(*vals, *vals_alt);
