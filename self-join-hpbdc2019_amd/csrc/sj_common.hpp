@@ -124,6 +124,9 @@ sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *i
                                     uint64_t *out /* n+1 */);
 sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
                                     uint32_t *out /* n+1 */);
+// exclusive scan of a[i] + b[i], total at out[n]
+sj_status scan_exclusive_sum2_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *a, const uint32_t *b,
+                                         uint64_t n, uint64_t *out /* n+1 */);
 // Stable LSD radix sort of (key,val) by the low `bits` bits of key.  The
 // sorted data ends in (*keys,*vals); the other buffers are scratch.
 sj_status radix_sort_pairs_u64(Pool *pool, hipStream_t s, uint64_t **keys, uint64_t **keys_alt,

@@ -1181,11 +1181,6 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 }
 
 // ----------------------------------------------------- small row kernels
-__global__ void k_totals_sorted(const uint32_t *own, const uint32_t *mir, uint64_t n, uint32_t *tot) {
-    uint64_t s = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (s >= n) return;
-    tot[s] = own[s] + mir[s];
-}
 
 // mirrored-hit cursors of rows [w0, w1): start right after the own hits
 __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
@@ -1808,12 +1803,8 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 
     // row offsets in cell order
     ts.start();
-    DBuf<uint32_t> tot;
-    SJ_TRY(tot.alloc(pool, n));
     SJ_TRY(r->d_offsets.alloc(pool, n + 1));
-    hipLaunchKernelGGL(k_totals_sorted, dim3(nblk(n)), dim3(JB), 0, s, own.p, mir.p, n, tot.p);
-    SJ_LAUNCH_CHECK();
-    SJ_TRY(scan_exclusive_u32_to_u64(pool, s, tot.p, n, r->d_offsets.p));
+    SJ_TRY(scan_exclusive_sum2_u32_to_u64(pool, s, own.p, mir.p, n, r->d_offsets.p));
     ts.stop();
     uint64_t *pin = ctx->pinned;
     SJ_TRY_HIP(hipMemcpyAsync(pin, r->d_offsets.p + n, 8, hipMemcpyDeviceToHost, s));
@@ -1868,9 +1859,13 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     double ms_fill = 0.0;
     if (whole) {
         SJ_TRY(r->d_ids.alloc(pool, total));
-        hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                           (uint32_t)n, 0ull, mirCur.p);
-        SJ_LAUNCH_CHECK();
+        // the replayed LDS tile walk places mirrored hits at the recorded
+        // reservations; every other fill takes them from the cursors
+        if (!(replay && a.mDP == 0)) {
+            hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                               (uint32_t)n, 0ull, mirCur.p);
+            SJ_LAUNCH_CHECK();
+        }
         a.ids = r->d_ids.p;
         a.outBase = 0;
         if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
