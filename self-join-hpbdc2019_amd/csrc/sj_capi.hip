@@ -250,6 +250,7 @@ SJ_API sj_status sj_self_join(sj_context ctx, const double *d_points, uint64_t n
     sj_grid g = nullptr;
     const auto t0 = std::chrono::steady_clock::now();
     SJ_TRY(sj_grid_build(ctx, d_points, n, d, k, eps, &g));
+    g->transient = true;
     sj_status s = sj_join(g, params, out);
     sj_grid_free(g);
     if (s == SJ_OK)

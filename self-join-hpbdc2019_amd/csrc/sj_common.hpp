@@ -173,6 +173,7 @@ struct sj_grid_s {
     sj::DBuf<uint32_t> posLB;        // totalCells+1 (dense grids): first sorted position with linear ID >= L
     sj::DBuf<float> sortedPts32;     // lazily built for the FP32 form
     double ms_build = 0.0;
+    bool transient = false;          // made for one join, which may take its buffers
 };
 
 struct sj_batch_host {

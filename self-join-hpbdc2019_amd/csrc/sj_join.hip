@@ -1181,6 +1181,18 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 }
 
 // ----------------------------------------------------- small row kernels
+// The count pass's counters at zero: mirrored-hit counts of the n rows,
+// statistics, and (when not null) the queue slice counters and the
+// hit-record counters.
+__global__ __launch_bounds__(JB) void k_zero_count_state(uint32_t *mir, uint64_t n, unsigned long long *stat,
+                                                         uint32_t *qctr, unsigned long long *recCtr) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (i < n) mir[i] = 0u;
+    if (i < 8) stat[i] = 0ull;
+    if (qctr && i < (uint64_t)QSLICES * QSTRIDE) qctr[i] = 0u;
+    if (recCtr && i < REC_NCTR) recCtr[i] = 0ull;
+}
+
 
 // mirrored-hit cursors of rows [w0, w1): start right after the own hits
 __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
@@ -1761,9 +1773,6 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(own.alloc(pool, n));
     SJ_TRY(mir.alloc(pool, n));
     SJ_TRY(stat.alloc(pool, 8));
-    SJ_TRY(fill_u32(s, mir.p, n, 0u));
-    SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p), 8, 0ull));
-    if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
     a.stat = stat.p;
@@ -1783,7 +1792,6 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             recording = false;  // no room: the fill evaluates distances again
             (void)hipGetLastError();
         } else {
-            SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(recCtr.p), REC_NCTR, 0ull));
             const uint32_t cw = a.mDP > 0 ? 32u : 64u;  // candidates per recorded piece
             hipLaunchKernelGGL(pick_tile_bound(g->k), dim3(nblk(tBound)), dim3(JB), 0, s, a, (uint32_t)tBound, cw,
                                tileBound.p);
@@ -1797,6 +1805,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             a.maskCap = maskCap;
         }
     }
+    const uint64_t nz = std::max<uint64_t>(n, std::max<uint64_t>(REC_NCTR, QSLICES * QSTRIDE));
+    hipLaunchKernelGGL(k_zero_count_state, dim3(nblk(nz)), dim3(JB), 0, s, mir.p, n, stat.p,
+                       a.tileOrder ? qctr.p : nullptr, recording ? recCtr.p : nullptr);
+    SJ_LAUNCH_CHECK();
     tc.start();
     SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();
@@ -1847,8 +1859,12 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         return SJ_ERR_STATE;
     }
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
-    SJ_TRY(r->d_rowIds.alloc(pool, n));
-    SJ_TRY_HIP(hipMemcpyAsync(r->d_rowIds.p, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
+    // row IDs: a grid made for this call alone hands its sorted IDs over at
+    // the end; otherwise they are copied
+    if (!g->transient) {
+        SJ_TRY(r->d_rowIds.alloc(pool, n));
+        SJ_TRY_HIP(hipMemcpyAsync(r->d_rowIds.p, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
+    }
 
     DBuf<unsigned long long> mirCur;
     SJ_TRY(mirCur.alloc(pool, n));
@@ -2015,6 +2031,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         r->stats.num_batches = win.size();
         r->stats.fill_from_record = replay ? 1 : 0;
     }
+    if (g->transient) r->d_rowIds = std::move(g->sortedIds);
     r->stats.ms_workload = tw.ms();
     r->stats.ms_count = tc.ms();
     r->stats.ms_scan = ts.ms();
