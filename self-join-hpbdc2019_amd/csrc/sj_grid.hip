@@ -175,7 +175,7 @@ template <class KT>
 __global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint32_t *head,
                                                       const uint32_t *ex, uint64_t n,
                                                       uint32_t *cellOf, uint32_t *cellStart,
-                                                      uint64_t *cellLinear) {
+                                                      uint64_t *cellLinear, uint32_t *nne) {
     uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
     if (i >= n) return;
     uint32_t c = ex[i] + head[i] - 1u;
@@ -184,7 +184,10 @@ __global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint
         cellStart[c] = (uint32_t)i;
         cellLinear[c] = (uint64_t)keys[i];
     }
-    if (i == n - 1) cellStart[c + 1] = (uint32_t)n;
+    if (i == n - 1) {
+        cellStart[c + 1] = (uint32_t)n;
+        *nne = c + 1;
+    }
 }
 
 // one thread per (point, coordinate): row s of the output is row ids[s]
@@ -311,18 +314,16 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
     SJ_TRY(g->cellStart.alloc(pool, n + 1));
     SJ_TRY(g->cellLinear.alloc(pool, n));
     hipLaunchKernelGGL((k_compact_cells<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, head.p, ex.p, n,
-                       g->cellOf.p, g->cellStart.p, g->cellLinear.p);
+                       g->cellOf.p, g->cellStart.p, g->cellLinear.p, d_flags + 1);
     SJ_LAUNCH_CHECK();
-    // sorted point IDs (indexLookupArr)
-    SJ_TRY(g->sortedIds.alloc(pool, n));
-    SJ_TRY_HIP(hipMemcpyAsync(g->sortedIds.p, vp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    // nNonEmpty and the error flags
+    // sorted point IDs (indexLookupArr): the sort's value buffer itself
+    g->sortedIds = std::move(vp == vals.p ? vals : vals2);
+    // the error flags and nNonEmpty (d_flags[1], written by the compaction)
     uint64_t *pin = g->ctx->pinned;
-    SJ_TRY_HIP(hipMemcpyAsync(pin, ex.p + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipMemcpyAsync(pin + 1, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipMemcpyAsync(pin, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SJ_TRY_HIP(hipStreamSynchronize(s));
-    uint32_t nne = (uint32_t)(pin[0] & 0xffffffffu);
-    uint32_t fl = (uint32_t)(pin[1] & 0xffffffffu);
+    const uint32_t fl = (uint32_t)(pin[0] & 0xffffffffu);
+    const uint32_t nne = (uint32_t)(pin[0] >> 32);
     if (fl & GP_OVERFLOW) {
         set_error("cell coordinate outside the grid (overflow)");
         return SJ_ERR_OVERFLOW;
@@ -357,7 +358,7 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     DBuf<double> partial;
     DBuf<uint32_t> flags;
     SJ_TRY(partial.alloc(pool, (uint64_t)nbmm * 2 * k));
-    SJ_TRY(flags.alloc(pool, 1));
+    SJ_TRY(flags.alloc(pool, 2));  // error flags, then the non-empty cell count
     SJ_TRY(g->d_gp.alloc(pool, 1));
     SJ_TRY(fill_u32(s, flags.p, 1, 0u));
     hipLaunchKernelGGL(k_minmax, dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);

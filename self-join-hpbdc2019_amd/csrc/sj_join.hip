@@ -1181,6 +1181,18 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 }
 
 // ----------------------------------------------------- small row kernels
+// The host's view after the count pass, gathered for one copy: out[0] =
+// |S|, out[1..8] = statistics, out[16] = record unusable, out[18] =
+// directory entries the record needed (the last two when recording).
+__global__ void k_count_scalars(const uint64_t *total, const unsigned long long *stat,
+                                const unsigned long long *recCtr, const uint64_t *dirTotal, uint64_t *out) {
+    const unsigned t = threadIdx.x;
+    if (t == 0) out[0] = *total;
+    if (t >= 1 && t <= 8) out[t] = stat[t - 1];
+    if (t == 16) out[16] = recCtr ? recCtr[REC_BAD] : 0ull;
+    if (t == 18) out[18] = dirTotal ? *dirTotal : 0ull;
+}
+
 // The count pass's counters at zero: mirrored-hit counts of the n rows,
 // statistics, and (when not null) the queue slice counters and the
 // hit-record counters.
@@ -1818,13 +1830,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(r->d_offsets.alloc(pool, n + 1));
     SJ_TRY(scan_exclusive_sum2_u32_to_u64(pool, s, own.p, mir.p, n, r->d_offsets.p));
     ts.stop();
+    // |S|, the statistics and the record's state in one readback
     uint64_t *pin = ctx->pinned;
-    SJ_TRY_HIP(hipMemcpyAsync(pin, r->d_offsets.p + n, 8, hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipMemcpyAsync(pin + 1, stat.p, 8 * 8, hipMemcpyDeviceToHost, s));
-    if (recording) {
-        SJ_TRY_HIP(hipMemcpyAsync(pin + 16, recCtr.p, 8, hipMemcpyDeviceToHost, s));
-        SJ_TRY_HIP(hipMemcpyAsync(pin + 18, tileDir.p + tBound, 8, hipMemcpyDeviceToHost, s));
-    }
+    DBuf<uint64_t> scal;
+    SJ_TRY(scal.alloc(pool, 19));
+    hipLaunchKernelGGL(k_count_scalars, dim3(1), dim3(64), 0, s, r->d_offsets.p + n, stat.p,
+                       recording ? recCtr.p : nullptr, recording ? tileDir.p + tBound : nullptr, scal.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, 19 * 8, hipMemcpyDeviceToHost, s));
     SJ_TRY_HIP(hipStreamSynchronize(s));
     bool replay = false;
     if (recording) {
