@@ -881,6 +881,14 @@ __global__ __launch_bounds__(JB) void k_join_rp(JoinArgs a) {
     join_body<K, D, PASS_FILL, PREC, true>(a, lds);
 }
 
+// k <= 2: five waves per SIMD (at most 96 VGPRs); measured faster at 2-D,
+// slower at 6-D
+template <int K, int D, int PREC>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(5))) void k_join_rp_o5(JoinArgs a) {
+    __shared__ TileLdsIds<D, PREC> lds[WPB];
+    join_body<K, D, PASS_FILL, PREC, true>(a, lds);
+}
+
 // ------------------------------------------------------------ many joins
 // One persistent launch over the tiles of many independent joins: work
 // item w is tile workTile[w] of instance workInst[w], whose arguments are
@@ -1542,7 +1550,8 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
 
 template <int K>
 static JoinKernel pick_replay_k(int prec) {
-    return prec ? k_join_rp<K, K, 1> : k_join_rp<K, K, 0>;
+    if constexpr (K <= 2) return prec ? k_join_rp_o5<K, K, 1> : k_join_rp_o5<K, K, 0>;
+    else return prec ? k_join_rp<K, K, 1> : k_join_rp<K, K, 0>;
 }
 
 static JoinKernel pick_replay(int k, int prec) {
