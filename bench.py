@@ -18,9 +18,15 @@ Workloads (--workload, default exp2d):
 One step = grid construction + count pass + offsets + fill pass over points
 already resident in HBM, producing the exact neighbour table in HBM (for
 many64: one replay of the recorded graph over the planned grids).  Prints
-one JSON line.  roofline covers the dominant kernel, timed with HIP events
-on the stream it runs on; cpu_baseline is the CPU oracle (grid join,
-OpenMP) on the host cores.
+one JSON line.  roofline.achieved / frac: the join's algorithmic bytes
+N*d*8 + |S|*4 + (N+1)*8 (FP32 matrix-core flops 2*d*C for clusters64d) over
+the step time; kernel_achieved / kernel_frac: the same work over the
+dominant kernel's time, from HIP events on the stream it runs on.
+cpu_baseline is the CPU oracle (grid join, OpenMP) on the host cores; the
+oracle's table is also the reference of config.exact_vs_oracle (an
+order-free fingerprint of every GPU table), and a mismatch exits 1.
+end_to_end times the reference's region: host points to host neighbour
+table.
 """
 import argparse
 import json
@@ -157,23 +163,37 @@ def canonical_pairs_C(g):
     return C
 
 
+def fingerprint_host(off, ids):
+    """fingerprint() of a CSR in original-ID order (rows are point IDs)."""
+    import numpy as np
+
+    rows = np.repeat(np.arange(off.size - 1, dtype=np.int64), np.diff(off.astype(np.int64)))
+    v = ids.astype(np.int64)
+    return (int(off[-1]), int((v * (rows + 1)).sum()), int((v * v).sum()))
+
+
 def cpu_baseline_join(points, k, eps, reps):
+    """The oracle's grid join (grid build + LID join, OpenMP on the host
+    cores), best of reps; also returns the fingerprint of its table, the
+    reference the GPU tables are checked against."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     cores = cores_used()
-    best, total = None, 0
+    best, total, fp = None, 0, None
     for _ in range(reps):
         t0 = time.perf_counter()
         g = O.OracleGrid(points, k, eps)
         off, ids = g.join(O.LID, nthreads=cores, sort_rows=False)
         dt = time.perf_counter() - t0
         total = int(off[-1])
+        if fp is None:
+            fp = fingerprint_host(off, ids)
         del g, off, ids
         best = dt if best is None else min(best, dt)
     return {"value": total / best, "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"full workload ({points.shape[0]} points, grid build + LID join), best of {reps}",
-            "seconds": best}
+            "seconds": best}, fp
 
 
 def cpu_baseline_sampled(points, k, eps, stride):
@@ -227,6 +247,7 @@ def bench_single(a, torch):
             g = ctx.grid_build(x, k, eps)
             C = canonical_pairs_C(g)
             g.free()
+        e2e = end_to_end(ctx, pts, k, eps, dict(kw, precision=sjamd.FP32_RECHECK), torch)
     exact = fp_a == fp_b
 
     dt, total, st = fp32
@@ -240,17 +261,24 @@ def bench_single(a, torch):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # join level: the algorithmic work of the whole join over the step time;
+    # kernel level: the same work over the dominant kernel's HIP-event time
     if a.workload == "clusters64d":
         flops = 2.0 * d * C
-        achieved = flops / (kms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / MFMA_FP32_PEAK_TFLOPS, "traffic": traffic,
-                "alg_flops": flops, "canonical_pairs_C": C, "kernel_ms": kms}
+        achieved = flops / dt / 1e12
+        kach = flops / (kms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MFMA_FP32_PEAK_TFLOPS, "traffic": traffic, "alg_flops": flops,
+                "canonical_pairs_C": C, "step_ms": ms,
+                "kernel": kname, "kernel_ms": kms, "kernel_achieved": kach,
+                "kernel_frac": kach / MFMA_FP32_PEAK_TFLOPS}
     else:
-        kbytes = n * d * 8 + total * 4 + (n + 1) * 8 if kname == "fill" else n * d * 8 + n * 4 * 2
-        achieved = kbytes / (kms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": kbytes, "kernel_ms": kms}
+        balg = n * d * 8 + total * 4 + (n + 1) * 8
+        achieved = balg / dt / 1e9
+        kach = balg / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": balg, "step_ms": ms,
+                "kernel": kname, "kernel_ms": kms, "kernel_achieved": kach, "kernel_frac": kach / HBM_PEAK_GBS}
     out = {
         "metric": METRIC,
         "value": total / dt,
@@ -276,15 +304,39 @@ def bench_single(a, torch):
                          "misclassified_before_recheck": st["fp32_misclassified"],
                          "candidates": st["candidates"], "fill_from_record": st["fill_from_record"]},
         "fp32_speedup_vs_fp64": fp64[0] / dt,
+        "end_to_end": e2e,
         "roofline": roof,
     }
+    out["config"]["exact_vs_oracle"] = None
     if not a.no_cpu:
         if a.workload == "clusters64d":
             out["cpu_baseline"] = cpu_baseline_sampled(pts, k, eps, 1000)
         else:
-            out["cpu_baseline"] = cpu_baseline_join(pts, k, eps, a.cpu_reps)
+            out["cpu_baseline"], fp_o = cpu_baseline_join(pts, k, eps, a.cpu_reps)
+            out["config"]["exact_vs_oracle"] = fp_a == fp_o and fp_b == fp_o
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
+
+
+def end_to_end(ctx, pts, k, eps, kw, torch, reps=3):
+    """The reference's timed region (main.cu:190-197: H2D, join, D2H and the
+    host neighbour table): host points -> device -> join -> the reference
+    neighbour table on the host (sj_result_neighbor_table: the table
+    downloaded into pinned memory and indexed per point).  Median of reps."""
+    times = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = ctx.to_device(pts)
+        r = ctx.self_join(x, k, eps, **kw)
+        tab = r.neighbor_table()
+        dt = time.perf_counter() - t0
+        r.free()
+        del x, tab
+        times.append(dt)
+    times = sorted(times[1:])
+    return {"ms": times[len(times) // 2] * 1e3,
+            "region": "host points -> H2D -> grid + join -> D2H of the table -> host neighbour table"}
 
 
 def bench_many(a, torch):
@@ -348,7 +400,7 @@ def bench_many(a, torch):
                                "Exp(40) data, 8..128 neighbours/pt, seed 5; one step = one graph replay "
                                "(count + offsets + fill of all 64 joins over the planned grids)",
                    "instances": len(batch), "points": int(sum(p.shape[0] for p, _, _ in batch)),
-                   "pairs": total, "exact_vs_fp64": exact, "autotuned": tune,
+                   "pairs": total, "exact_vs_fp64": exact, "exact_vs_oracle": None, "autotuned": tune,
                    "precision": "fp32+fp64-recheck"},
         "plan_ms": f32["times"]["ms_plan"], "plan_grid_ms": f32["times"]["ms_grid"],
         "fp64": {"ms_per_step": res["fp64"]["dt"] * 1e3, "pairs_per_s": res["fp64"]["total"] / res["fp64"]["dt"],
@@ -358,21 +410,28 @@ def bench_many(a, torch):
                          "misclassified_before_recheck": f32["stats"]["fp32_misclassified"],
                          "candidates": f32["stats"]["candidates"]},
         "fp32_speedup_vs_fp64": res["fp64"]["dt"] / f32["dt"],
-        "roofline": {"bound": "hbm", "kernel": "graph replay (count + offsets + fill, all instances)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes": kbytes, "kernel_ms": kms},
+        "roofline": {"bound": "hbm", "achieved": kbytes / f32["dt"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kbytes / f32["dt"] / 1e9 / HBM_PEAK_GBS, "traffic": None, "alg_bytes": kbytes,
+                     "step_ms": f32["dt"] * 1e3,
+                     "kernel": "graph replay (count + offsets + fill, all instances)", "kernel_ms": kms,
+                     "kernel_achieved": achieved, "kernel_frac": achieved / HBM_PEAK_GBS},
     }
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
         cores = cores_used()
-        t0 = time.perf_counter()
         tot = 0
+        sec = 0.0
+        fps_o = []
         for p, k, e in batch:
-            off, _ = O.OracleGrid(p, k, e).join(O.LID, nthreads=cores, sort_rows=False)
+            t0 = time.perf_counter()
+            off, ids = O.OracleGrid(p, k, e).join(O.LID, nthreads=cores, sort_rows=False)
+            sec += time.perf_counter() - t0
             tot += int(off[-1])
-        sec = time.perf_counter() - t0
+            fps_o.append(fingerprint_host(off, ids))
+            del off, ids
+        out["config"]["exact_vs_oracle"] = fps_o == res["fp32"]["fps"] and fps_o == res["fp64"]["fps"]
         out["cpu_baseline"] = {"value": tot / sec, "unit": "pairs/s", "cores": cores, "kind": "port",
                                "sample": "all 64 joins (grid build + LID join each), one pass", "seconds": sec}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
@@ -467,6 +526,9 @@ def main():
         return
     out = bench_many(a, torch) if a.workload == "many64" else bench_single(a, torch)
     print(json.dumps(out))
+    if out["config"].get("exact_vs_oracle") is False:
+        print("bench.py: the GPU neighbour table differs from the oracle's", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

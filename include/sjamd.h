@@ -5,6 +5,13 @@
  * call, no exits.  Device pointers are HIP device addresses; `stream` is a
  * hipStream_t passed as void* (NULL: the context creates its own).
  *
+ * Ordering: every entry point enqueues its work on the context's stream
+ * and reads the caller's device buffers there.  Input produced on another
+ * stream must be complete, or that stream joined to the context's stream
+ * (an event wait), before the call.  A context created with stream = NULL
+ * uses a blocking stream, so work the caller enqueued earlier on the legacy
+ * default stream (stream 0) is complete before the context's work starts.
+ *
  * Each entry point names the reference (benoitgallet/self-join-hpbdc2019)
  * interface it replaces.
  */
@@ -46,11 +53,15 @@ typedef enum sj_precision {
     SJ_PREC_FP32_RECHECK = 1 /* FP32 bulk + FP64 re-decision of the margin band */
 } sj_precision;
 
-/* Query-point scheduling: strided batches (modes 3-6, kernel.cu:978) or the
- * workload-sorted atomic work queue (modes 9-10, kernel.cu:2202-2236). */
+/* Query-point scheduling: strided batches (modes 3-6, kernel.cu:978), the
+ * workload-sorted atomic work queue (modes 7, 9, 10, kernel.cu:2202-2236),
+ * or several lanes per query point over the workload order (mode 8,
+ * kernel.cu:2072-2188: 16, 8 and 2 lanes per point for the heaviest,
+ * middle and lightest third; full stencil and FP64 only). */
 typedef enum sj_schedule {
     SJ_SCHED_STRIDED = 0,
-    SJ_SCHED_WORKQUEUE = 1
+    SJ_SCHED_WORKQUEUE = 1,
+    SJ_SCHED_THREADS_PER_POINT = 2
 } sj_schedule;
 
 typedef struct sj_context_s *sj_context;
@@ -180,10 +191,24 @@ SJ_API void sj_result_free(sj_result r);
 SJ_API sj_status sj_workload(sj_grid g, int32_t lid, int64_t *nbPoints, uint32_t *order);
 
 /* Batch estimator (callGPUBatchEst / callGPUBatchEstWorkQueue, GPU.h:42-50,
- * GPU.cu:854-1284): sampled count and the buffer/batch policy. */
+ * GPU.cu:854-1284): full-stencil hit count of every stride-th point
+ * (stride 0: 100, the reference's sample rate), estimated = count * stride,
+ * estimatedWithAlpha = estimated * 1.05 (the value callGPUBatchEst returns),
+ * and the buffer/batch policy of GPU.cu:1029-1055 (numBatches, bufferSize =
+ * 1.5x the per-batch buffer).  Any output pointer may be NULL. */
 SJ_API sj_status sj_batch_estimate(sj_grid g, uint64_t stride, uint64_t *sampleCount,
-                                   uint64_t *estimated, uint64_t *numBatches,
-                                   uint64_t *bufferSize);
+                                   uint64_t *estimated, uint64_t *estimatedWithAlpha,
+                                   uint64_t *numBatches, uint64_t *bufferSize);
+/* Work-queue estimator (callGPUBatchEstWorkQueue, GPU.h:47-50, GPU.cu:1072-1284,
+ * kernel.cu:785-942): the sample is the first n/stride points of the
+ * workload-ordered point list (cells by descending workload, ties by
+ * ascending cell index; points of a cell in ascending ID), the order modes 9
+ * (lid = 0, full-stencil workload) and 10 (lid = 1, LID workload) run in
+ * (GPU.cu:1685-1692).  Same outputs and policy as sj_batch_estimate. */
+SJ_API sj_status sj_batch_estimate_workqueue(sj_grid g, int32_t lid, uint64_t stride,
+                                             uint64_t *sampleCount, uint64_t *estimated,
+                                             uint64_t *estimatedWithAlpha, uint64_t *numBatches,
+                                             uint64_t *bufferSize);
 
 /* Replaces importNDDataset (import_dataset.cpp:34-73).  *points is
  * allocated with malloc; release with sj_free_host. */
@@ -242,6 +267,26 @@ SJ_API void sj_many_free(sj_many m);
 
 /* Eps threshold T = max{x : sqrt_rn(x) <= eps}; sqrt(d2) <= eps iff d2 <= T. */
 SJ_API double sj_eps_threshold(double eps);
+
+/* Band of the FP32 precision form on the VALU tile walk (no reference
+ * counterpart; exposed so the bound can be checked on the host).  Each
+ * coordinate is stored as y = fl32(x - c) with |x - c| <= R for the tile's
+ * FP64 centre c; s32 is the FMA sum of the D squared FP32 differences in
+ * dimension order.  Then d2_64 <= T implies s32 <= hi and d2_64 > T implies
+ * s32 >= lo, so s32 < lo is a hit, s32 > hi a miss, and [lo, hi] is
+ * re-decided in FP64.  t32 = fl32(T) is the plain FP32 decision whose
+ * errors sj_stats.fp32_misclassified counts.  Returns 1 when the band is
+ * usable (narrow enough), else 0 (the join then runs the FP64 predicate). */
+SJ_API int32_t sj_fp32_band(double T, double R, int32_t D, float *lo, float *hi, float *t32);
+
+/* Decision constants of the FP32 form on the matrix cores (no reference
+ * counterpart).  With FP32 vectors centred on the tile's first point, s =
+ * |q|^2 + |c|^2, d2 = s - 2 q.c (any summation order) and M = alpha s +
+ * beta: d2 + M <= tlo implies d2_64 <= T, d2 - M > thi implies d2_64 > T.
+ * R >= 0: staging from the FP32 copy relative to the grid origin, whose
+ * vectors are within R of it (Euclidean); R < 0: centring in FP64.
+ * Returns the padded operand width (4..64), 0 when d is above 64. */
+SJ_API int32_t sj_mfma_band(double T, double R, int32_t d, float *alpha, float *beta, float *tlo, float *thi);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,7 @@ def lib():
         L.sjo_eps_threshold.argtypes = [C.c_double]
         L.sjo_dist2.restype = C.c_double
         L.sjo_dist2.argtypes = [dp, dp, C.c_int]
+        L.sjo_dist2_batch.argtypes = [dp, dp, C.c_uint64, C.c_int, dp]
         L.sjo_linear_id.restype = C.c_uint64
         L.sjo_grid_dims.argtypes = [dp, C.c_uint64, C.c_int, C.c_int, C.c_double,
                                     dp, dp, C.POINTER(C.c_uint32), u64p]
@@ -73,6 +74,8 @@ def lib():
         L.sjo_workload_order.argtypes = [C.POINTER(Grid), C.c_int, C.POINTER(C.c_uint32)]
         L.sjo_sample_count.restype = C.c_uint64
         L.sjo_sample_count.argtypes = [C.POINTER(Grid), dp, C.c_uint64]
+        L.sjo_sample_count_workqueue.restype = C.c_uint64
+        L.sjo_sample_count_workqueue.argtypes = [C.POINTER(Grid), dp, C.c_uint64, C.c_int]
         L.sjo_batch_policy.argtypes = [C.c_uint64, C.c_uint64, u64p, u64p, u64p, u64p]
         L.sjo_neighbor_table.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.c_uint64, C.c_int32, C.POINTER(NTL)]
@@ -102,6 +105,15 @@ def dist2(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     return lib().sjo_dist2(_dptr(a), _dptr(b), a.size)
+
+
+def dist2_batch(a, b):
+    """Predicate P's squared distance of each row pair of a and b (m, d)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty(a.shape[0])
+    lib().sjo_dist2_batch(_dptr(a), _dptr(b), a.shape[0], a.shape[1], _dptr(out))
+    return out
 
 
 class OracleError(RuntimeError):
@@ -180,6 +192,10 @@ class OracleGrid:
 
     def sample_count(self, stride=100):
         return lib().sjo_sample_count(C.byref(self.g), _dptr(self.points), stride)
+
+    def sample_count_workqueue(self, stride=100, lid=False):
+        return lib().sjo_sample_count_workqueue(C.byref(self.g), _dptr(self.points), stride,
+                                                1 if lid else 0)
 
 
 def join(points, k, eps, form=LID, nthreads=0, sort_rows=True):

@@ -68,6 +68,12 @@ double sjo_dist2(const double *a, const double *b, int d)
     return acc;
 }
 
+/* Predicate P over m pairs: out[i] = sjo_dist2(a + i*d, b + i*d, d). */
+void sjo_dist2_batch(const double *a, const double *b, uint64_t m, int d, double *out)
+{
+    for (uint64_t i = 0; i < m; i++) out[i] = sjo_dist2(a + i * d, b + i * d, d);
+}
+
 /* ------------------------------------------------------------------ */
 /* G2: main.cu:423-439 (dim 0 fastest).                                */
 /* ------------------------------------------------------------------ */
@@ -572,6 +578,50 @@ uint64_t sjo_sample_count(const sjo_grid *g, const double *pts, uint64_t stride)
         }
     }
     free(L.dense); free(S.off);
+    return total;
+}
+
+/* B1, work-queue form: kernel.cu:785-942 takes point IDs 0..nsamp-1 of   */
+/* the workload-ordered point list sortedCells (GPU.cu:1697-1716: cells  */
+/* by descending workload, points of a cell in index order) through the  */
+/* counter counterEstimator; nsamp = N * 0.01 (GPU.cu:1104-1106).  The    */
+/* ordering workload is the LID one for modes 6 and 10, else the full    */
+/* one (GPU.cu:1685-1692).  Cell ties are broken by ascending cell index. */
+uint64_t sjo_sample_count_workqueue(const sjo_grid *g, const double *pts, uint64_t stride, int lid)
+{
+    if (stride == 0) stride = 1;
+    const uint64_t nsamp = (uint64_t)((double)g->n * (1.0 / (double)stride));
+    if (nsamp == 0) return 0;
+    uint32_t *order = malloc((g->nNonEmpty ? g->nNonEmpty : 1) * sizeof(uint32_t));
+    uint32_t *list = malloc(nsamp * sizeof(uint32_t));
+    if (!order || !list) { free(order); free(list); return 0; }
+    if (sjo_workload_order(g, lid, order)) { free(order); free(list); return 0; }
+    uint64_t r = 0;
+    for (uint64_t i = 0; i < g->nNonEmpty && r < nsamp; i++)
+        for (int32_t j = g->indexMin[order[i]]; j <= g->indexMax[order[i]] && r < nsamp; j++)
+            list[r++] = g->indexLookup[j];
+    lookup_t L;
+    stencil_t S;
+    if (lookup_init(&L, g)) { free(order); free(list); return 0; }
+    if (stencil_init(&S, g->k)) { free(L.dense); free(order); free(list); return 0; }
+    const int d = g->d, k = g->k;
+    const double T = sjo_eps_threshold(g->eps);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int64_t t = 0; t < (int64_t)nsamp; t++) {
+        const double *x = pts + (uint64_t)list[t] * d;
+        uint32_t c[SJO_MAXK];
+        for (int j = 0; j < k; j++) cell_coord(x[j], g->minArr[j], g->eps, &c[j]);
+        int zero[SJO_MAXK] = {0};
+        for (uint64_t s = 0; s <= S.nOff; s++) {
+            const int *o = s < S.nOff ? S.off + s * k : zero;
+            int64_t b = neighbor(&L, c, o);
+            if (b < 0) continue;
+            for (int32_t j = g->indexMin[b]; j <= g->indexMax[b]; j++)
+                total += sjo_dist2(x, pts + (uint64_t)g->indexLookup[j] * d, d) <= T;
+        }
+    }
+    free(L.dense); free(S.off); free(order); free(list);
     return total;
 }
 

@@ -43,6 +43,7 @@ typedef struct {
 /* Predicate P */
 double sjo_eps_threshold(double eps);
 double sjo_dist2(const double *a, const double *b, int d);
+void sjo_dist2_batch(const double *a, const double *b, uint64_t m, int d, double *out);
 
 /* G2 */
 uint64_t sjo_linear_id(const uint32_t *idx, const uint32_t *nCells, int k);
@@ -80,6 +81,9 @@ int sjo_workload_order(const sjo_grid *g, int lid, uint32_t *order);
 
 /* B1: sampled estimate (every 100th point, full stencil). */
 uint64_t sjo_sample_count(const sjo_grid *g, const double *pts, uint64_t stride);
+/* B1, work-queue form: the first n/stride points of the workload-ordered
+ * point list (sjo_workload_order with lid, points of a cell in ID order). */
+uint64_t sjo_sample_count_workqueue(const sjo_grid *g, const double *pts, uint64_t stride, int lid);
 void sjo_batch_policy(uint64_t sampleCount, uint64_t offsetRate,
                       uint64_t *estimated, uint64_t *estimatedWithAlpha,
                       uint64_t *numBatches, uint64_t *bufferSize);

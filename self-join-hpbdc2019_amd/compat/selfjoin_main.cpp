@@ -1,11 +1,16 @@
 // Driver with the reference program's flow (main.cu:36-226): import,
-// bin pre-sort, grid dimensions, grid index, batched join, neighbour table.
-// Usage: sjamd_selfjoin dataset eps ndim searchmode [nindexed] [dump.txt]
-// The optional dump lists every point's neighbours, sorted, one line each.
+// bin pre-sort, grid dimensions, grid index, batched join, neighbour table,
+// the per-mode stats line and the neighbour-table listing.
+// Usage: sjamd_selfjoin dataset eps ndim searchmode [nindexed] [table.txt]
+// The stats line is appended to gpu_<mode>_stats.txt in the working
+// directory (main.cu:88-150,203-208).  The listing (main.cu:212-222) goes to
+// table.txt when given (the reference prints it to stdout, which at 2M
+// points is gigabytes of text).
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <vector>
 
 #include "sjamd_compat.hpp"
@@ -13,7 +18,7 @@
 int main(int argc, char *argv[]) {
     if (argc < 5) {
         printf("\n\nIncorrect number of input parameters.  \nShould be dataset file, epsilon, number of "
-               "dimensions, searchmode [, indexed dimensions, dump file]\n");
+               "dimensions, searchmode [, indexed dimensions, table file]\n");
         return 1;
     }
     const double epsilon = atof(argv[2]);
@@ -22,11 +27,21 @@ int main(int argc, char *argv[]) {
     const int nidx = argc > 5 ? atoi(argv[5]) : ndim;
     const char *dump = argc > 6 ? argv[6] : nullptr;
     sjamdSetDims(ndim, nidx);
+    printf("\nDataset file: %s", argv[1]);
+    printf("\nEpsilon: %f", epsilon);
+    printf("\nNumber of dimensions (NDIM): %d\n", ndim);
 
     std::vector<std::vector<DTYPE>> pts;
     importNDDataset(&pts, argv[1]);
     if (sjamdLastStatus() || pts.empty()) return 2;
     sortInNDBins(&pts);
+
+    std::ofstream gpu_stats;
+    const char *stats_name = searchModeStatsFile(mode);
+    if (stats_name) {
+        printf("\n\n# Search mode is : %s #\n\n", searchModeName(mode));
+        gpu_stats.open(stats_name, std::ios::app);
+    }
     warmUpGPU();
 
     std::vector<DTYPE> minArr(nidx), maxArr(nidx);
@@ -52,22 +67,18 @@ int main(int argc, char *argv[]) {
                                indexLookupArr.data(), table.data(), &batches, &totalNeighbors, mask,
                                maskOffsets.data(), &nMask);
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    printf("\nTime: %f\ntotalNeighbors: %llu\nbatches: %zu\n", t, (unsigned long long)totalNeighbors,
-           batches.size());
+    printf("\n~~~Time: %f", t);
+    printf("\ntotalNeighbors: %llu\nbatches: %zu\n", (unsigned long long)totalNeighbors, batches.size());
     if (sjamdLastStatus()) return 3;
+    if (stats_name) {
+        writeStatsLine(gpu_stats, t, argv[1], epsilon, totalNeighbors);
+        gpu_stats.close();
+    }
 
     if (dump) {
         FILE *f = fopen(dump, "w");
         if (!f) return 4;
-        for (size_t i = 0; i < pts.size(); i++) {
-            std::vector<int> v;
-            if (table[i].dataPtr)
-                v.assign(table[i].dataPtr + table[i].indexmin, table[i].dataPtr + table[i].indexmax + 1);
-            std::sort(v.begin(), v.end());
-            fprintf(f, "%zu:", i);
-            for (int x : v) fprintf(f, " %d", x);
-            fprintf(f, "\n");
-        }
+        printNeighborTable(f, table.data(), pts.size());
         fclose(f);
     }
     for (auto &b : batches) delete[] b.dataPtr;

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ostream>
 #include <vector>
 
 #include "sjamd.h"
@@ -177,7 +178,10 @@ void distanceTableNDGridBatches(int searchMode, std::vector<std::vector<DTYPE>> 
     case 4: P.form = SJ_FORM_UNICOMP; P.schedule = SJ_SCHED_STRIDED; break;
     case 5: P.form = SJ_FORM_BUNICOMP; P.schedule = SJ_SCHED_STRIDED; break;
     case 6: P.form = SJ_FORM_LID; P.schedule = SJ_SCHED_STRIDED; break;
-    case 7: case 8: case 9: P.form = SJ_FORM_FULL; P.schedule = SJ_SCHED_WORKQUEUE; break;
+    // 7: full stencil over the workload-sorted points (here through the work
+    // queue); 8: several lanes per point; 9: the work queue
+    case 7: case 9: P.form = SJ_FORM_FULL; P.schedule = SJ_SCHED_WORKQUEUE; break;
+    case 8: P.form = SJ_FORM_FULL; P.schedule = SJ_SCHED_THREADS_PER_POINT; break;
     case 10: P.form = SJ_FORM_LID; P.schedule = SJ_SCHED_WORKQUEUE; break;
     default:
         fprintf(stderr, "\nError: search mode %d is not in 3..10\n", searchMode);
@@ -188,9 +192,14 @@ void distanceTableNDGridBatches(int searchMode, std::vector<std::vector<DTYPE>> 
     DevicePoints dp(*NDdataPoints);
     sj_grid g = build_grid(NDdataPoints, *epsilon, dp);
     if (!g) return;
-    // result buffer size from the reference estimator (GPU.cu:854-1065)
-    uint64_t cnt = 0, est = 0, nb = 0, buf = 0;
-    if (fail("sj_batch_estimate", sj_batch_estimate(g, 100, &cnt, &est, &nb, &buf))) {
+    // result buffer size from the reference estimators: the work-queue one
+    // for modes 9 and 10, the strided sample otherwise (GPU.cu:1748-1758)
+    uint64_t cnt = 0, est = 0, esta = 0, nb = 0, buf = 0;
+    const sj_status est_st =
+        (searchMode == 9 || searchMode == 10)
+            ? sj_batch_estimate_workqueue(g, searchMode == 10, 100, &cnt, &est, &esta, &nb, &buf)
+            : sj_batch_estimate(g, 100, &cnt, &est, &esta, &nb, &buf);
+    if (fail("sj_batch_estimate", est_st)) {
         sj_grid_free(g);
         return;
     }
@@ -237,6 +246,39 @@ void distanceTableNDGridBatches(int searchMode, std::vector<std::vector<DTYPE>> 
     g_status = 0;
 }
 
+namespace {
+unsigned long long batch_est(std::vector<std::vector<DTYPE>> *NDdataPoints, DTYPE epsilon, int workqueue, int lid,
+                             unsigned int *retNumBatches, unsigned int *retGPUBufferSize) {
+    DevicePoints dp(*NDdataPoints);
+    sj_grid g = build_grid(NDdataPoints, epsilon, dp);
+    if (!g) return 0;
+    uint64_t cnt = 0, est = 0, esta = 0, nb = 0, buf = 0;
+    const sj_status st = workqueue ? sj_batch_estimate_workqueue(g, lid, 100, &cnt, &est, &esta, &nb, &buf)
+                                   : sj_batch_estimate(g, 100, &cnt, &est, &esta, &nb, &buf);
+    sj_grid_free(g);
+    if (fail("sj_batch_estimate", st)) return 0;
+    printf("\nGPU: result set size for estimating the number of batches (sampled): %llu",
+           (unsigned long long)cnt);
+    printf("\nNumber of batches: %llu, buffer size: %llu", (unsigned long long)nb, (unsigned long long)buf);
+    if (retNumBatches) *retNumBatches = (unsigned int)nb;
+    if (retGPUBufferSize) *retGPUBufferSize = (unsigned int)buf;
+    g_status = 0;
+    return esta;
+}
+}  // namespace
+
+unsigned long long callGPUBatchEst(std::vector<std::vector<DTYPE>> *NDdataPoints, DTYPE epsilon,
+                                   unsigned int *retNumBatches, unsigned int *retGPUBufferSize) {
+    return batch_est(NDdataPoints, epsilon, 0, 0, retNumBatches, retGPUBufferSize);
+}
+
+unsigned long long callGPUBatchEstWorkQueue(std::vector<std::vector<DTYPE>> *NDdataPoints, DTYPE epsilon,
+                                            int searchMode, unsigned int *retNumBatches,
+                                            unsigned int *retGPUBufferSize) {
+    return batch_est(NDdataPoints, epsilon, 1, searchMode == 6 || searchMode == 10 ? 1 : 0, retNumBatches,
+                     retGPUBufferSize);
+}
+
 // GPU.cu:2595-2637, without reading pointIDKey[cnt].
 void constructNeighborTableKeyValueWithPtrs(int *pointIDKey, int *pointInDistValue,
                                             struct neighborTableLookup *neighborTable, int *pointersToNeighbors,
@@ -253,4 +295,52 @@ void constructNeighborTableKeyValueWithPtrs(int *pointIDKey, int *pointInDistVal
         neighborTable[key].dataPtr = pointersToNeighbors;
         i = j + 1;
     }
+}
+
+namespace {
+struct ModeText {
+    int mode;
+    const char *file, *name;
+};
+// main.cu:88-150
+const ModeText kModes[] = {
+    {3, "gpu_global_stats.txt", "global memory kernel"},
+    {4, "gpu_unicomp_stats.txt", "Unicomp"},
+    {5, "gpu_b-unicomp_stats.txt", "B-Unicomp"},
+    {6, "gpu_linID-unicomp_stats.txt", "Linear ID Unicomp"},
+    {7, "gpu_sorting-cells_stats.txt", "sorting cells by workload"},
+    {8, "gpu_dynamic-threads-fixed_stats.txt", "dynamic threads per points (fixed)"},
+    {9, "gpu_working-queue_stats.txt", "working queue"},
+    {10, "gpu_working-queue_lid-unicomp_stats.txt", "working queue + LID-Unicomp"},
+};
+}  // namespace
+
+const char *searchModeStatsFile(int searchMode) {
+    for (const auto &m : kModes)
+        if (m.mode == searchMode) return m.file;
+    return nullptr;
+}
+
+const char *searchModeName(int searchMode) {
+    for (const auto &m : kModes)
+        if (m.mode == searchMode) return m.name;
+    return nullptr;
+}
+
+void writeStatsLine(std::ostream &out, double totalTime, const char *inputFname, DTYPE epsilon,
+                    uint64_t totalNeighbors) {
+    out << totalTime << ", " << inputFname << ", " << epsilon << ", " << totalNeighbors
+        << ", GPUNUMDIM/NUMINDEXEDDIM/DTYPE/BLOCKSIZE/THperPOINT/SORT: " << g_d << ", " << g_k << ", "
+        << "double" << ", " << 256 << ", " << 1 << ", " << 1 << std::endl;
+}
+
+void printNeighborTable(FILE *out, struct neighborTableLookup *neighborTable, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        neighborTableLookup &t = neighborTable[i];
+        if (t.dataPtr && t.indexmax >= t.indexmin) std::sort(t.dataPtr + t.indexmin, t.dataPtr + t.indexmax + 1);
+        fprintf(out, "\npoint id: %d, neighbors: ", (int)i);
+        fprintf(out, "nb neighbor %d\n", t.indexmax - t.indexmin + 1);
+        for (int j = t.indexmin; j < t.indexmax; j++) fprintf(out, "%d,", t.dataPtr[j]);
+    }
+    fprintf(out, "\n\n\n");
 }

@@ -13,6 +13,8 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
+#include <ostream>
 #include <vector>
 
 #ifndef DTYPE
@@ -90,6 +92,19 @@ void distanceTableNDGridBatches(int searchMode, std::vector<std::vector<DTYPE>> 
                                 uint64_t *totalNeighbors, unsigned int *gridCellNDMask,
                                 unsigned int *gridCellNDMaskOffsets, unsigned int *nNDMaskElems);
 
+// GPU.h:42-45 (callGPUBatchEst): sampled result-size estimate of every
+// 100th point; returns the estimate with alpha and sets the batch count and
+// the 1.5x buffer size (GPU.cu:1029-1063).  The reference's device-array
+// arguments are replaced by the points: the device builds its own grid.
+unsigned long long callGPUBatchEst(std::vector<std::vector<DTYPE>> *NDdataPoints, DTYPE epsilon,
+                                   unsigned int *retNumBatches, unsigned int *retGPUBufferSize);
+// GPU.h:47-50 (callGPUBatchEstWorkQueue): the sample is the first 1% of the
+// workload-ordered points (LID workload for searchMode 6 and 10, full
+// otherwise, GPU.cu:1685-1692).
+unsigned long long callGPUBatchEstWorkQueue(std::vector<std::vector<DTYPE>> *NDdataPoints, DTYPE epsilon,
+                                            int searchMode, unsigned int *retNumBatches,
+                                            unsigned int *retGPUBufferSize);
+
 // GPU.h:52: run-length neighbour table over key-sorted (key,value) pairs.
 void constructNeighborTableKeyValueWithPtrs(int *pointIDKey, int *pointInDistValue,
                                             struct neighborTableLookup *neighborTable, int *pointersToNeighbors,
@@ -97,3 +112,22 @@ void constructNeighborTableKeyValueWithPtrs(int *pointIDKey, int *pointInDistVal
 
 // GPU.h:54
 void warmUpGPU();
+
+// ---- the reference program's outputs (main.cu:88-150, 203-222) --------
+// Stats file of a search mode ("gpu_b-unicomp_stats.txt" for mode 5, ...),
+// NULL outside 3..10.
+const char *searchModeStatsFile(int searchMode);
+// Banner text of a search mode ("B-Unicomp" for mode 5, ...), NULL outside 3..10.
+const char *searchModeName(int searchMode);
+// One stats line (main.cu:203-206): time, dataset, epsilon, |S|, then
+// GPUNUMDIM, NUMINDEXEDDIM, DTYPE, BLOCKSIZE, THREADPERPOINT and
+// SORT_BY_WORKLOAD as the reference build reports them (params.h defaults:
+// 256, 1, 1).
+void writeStatsLine(std::ostream &out, double totalTime, const char *inputFname, DTYPE epsilon,
+                    uint64_t totalNeighbors);
+// Neighbour-table listing (main.cu:212-222): every point's neighbours
+// sorted in place, then "\npoint id: i, neighbors: nb neighbor c\n" and the
+// IDs at positions indexmin .. indexmax-1, each followed by a comma (the
+// reference's loop stops one short of indexmax, so the last, largest
+// neighbour is not listed).
+void printNeighborTable(FILE *out, struct neighborTableLookup *neighborTable, size_t n);

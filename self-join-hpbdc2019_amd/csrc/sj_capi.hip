@@ -6,6 +6,7 @@
 #include <new>
 
 #include "sj_common.hpp"
+#include "sj_traverse.hpp"
 
 namespace sj {
 const char *last_error();
@@ -15,6 +16,7 @@ sj_status grid_nd_mask_host(sj_grid_s *g, std::vector<uint32_t> &mask, std::vect
 sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r);
 sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order);
 sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count);
+sj_status batch_sample_count_wq(sj_grid_s *g, int lid, uint64_t stride, uint64_t *count);
 double eps_threshold(double eps);
 sj_status import_csv(const char *path, int d, double **pts, uint64_t *n);
 sj_status sort_in_nd_bins(double *pts, uint64_t n, int d, int stable);
@@ -77,6 +79,24 @@ SJ_API const char *sj_last_error(void) { return sj::last_error(); }
 
 SJ_API double sj_eps_threshold(double eps) { return sj::eps_threshold(eps); }
 
+SJ_API int32_t sj_fp32_band(double T, double R, int32_t D, float *lo, float *hi, float *t32) {
+    const Fp32Margin m = fp32_margin(T, R, D);
+    if (lo) *lo = m.lo;
+    if (hi) *hi = m.hi;
+    if (t32) *t32 = m.t32;
+    return m.usable ? 1 : 0;
+}
+
+SJ_API int32_t sj_mfma_band(double T, double R, int32_t d, float *alpha, float *beta, float *tlo, float *thi) {
+    if (mfma_padded_dims(d) == 0) return 0;
+    const MfmaBand b = mfma_band(T, R, d);
+    if (alpha) *alpha = b.alpha;
+    if (beta) *beta = b.beta;
+    if (tlo) *tlo = b.tlo;
+    if (thi) *thi = b.thi;
+    return b.DP;
+}
+
 SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out) {
     if (!out) {
         set_error("sj_context_create: null output");
@@ -97,7 +117,9 @@ SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out
         c->stream = static_cast<hipStream_t>(stream);
         c->own_stream = false;
     } else {
-        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        // a blocking stream: its work is ordered after earlier work of the
+        // legacy default stream, where a caller may have produced d_points
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
         if (e != hipSuccess) {
             delete c;
             set_error("hipStreamCreate: %s", hipGetErrorString(e));
@@ -236,7 +258,9 @@ SJ_API sj_status sj_join(sj_grid g, const sj_join_params *params, sj_result *out
     if (!r) return SJ_ERR_NOMEM;
     sj_status s = join(g, params, r);
     if (s != SJ_OK) {
+        // no kernel or batch download may still touch the result's buffers
         (void)hipStreamSynchronize(g->ctx->stream);
+        if (g->ctx->copyStream) (void)hipStreamSynchronize(g->ctx->copyStream);
         delete r;
         return s;
     }
@@ -386,8 +410,30 @@ SJ_API sj_status sj_workload(sj_grid g, int32_t lid, int64_t *nbPoints, uint32_t
     return grid_workload(g, lid, nbPoints, order);
 }
 
+// Buffer/batch policy of GPU.cu:1029-1055 (callGPUBatchEst) and
+// GPU.cu:1245-1271 (callGPUBatchEstWorkQueue): 3 streams, a 40M-pair
+// buffer, alpha 0.05 (2 alpha and a third of the estimate per buffer when
+// the estimate is below three buffers), buffer returned 1.5x.  An estimate
+// of 0 gives one batch of one pair (the reference divides by zero there).
+static void batch_policy(uint64_t cnt, uint64_t stride, uint64_t *sampleCount, uint64_t *estimated,
+                         uint64_t *estimatedWithAlpha, uint64_t *numBatches, uint64_t *bufferSize) {
+    const uint64_t est = cnt * stride;
+    uint32_t buf = 40000000u;
+    const double alpha = 0.05;
+    const uint64_t withAlpha = (uint64_t)((double)est * (1.0 + alpha * 1.0));
+    if (est < (uint64_t)buf * 3u) buf = (uint32_t)((double)est * (1.0 + alpha * 2.0) / 3.0);
+    if (buf == 0) buf = 1;
+    uint64_t nb = (uint64_t)std::ceil(((1.0 + alpha) * (double)est) / (double)buf);
+    if (nb == 0) nb = 1;
+    if (sampleCount) *sampleCount = cnt;
+    if (estimated) *estimated = est;
+    if (estimatedWithAlpha) *estimatedWithAlpha = withAlpha;
+    if (numBatches) *numBatches = nb;
+    if (bufferSize) *bufferSize = (uint64_t)(uint32_t)(1.5 * (double)buf);
+}
+
 SJ_API sj_status sj_batch_estimate(sj_grid g, uint64_t stride, uint64_t *sampleCount, uint64_t *estimated,
-                                   uint64_t *numBatches, uint64_t *bufferSize) {
+                                   uint64_t *estimatedWithAlpha, uint64_t *numBatches, uint64_t *bufferSize) {
     if (!g) {
         set_error("sj_batch_estimate: null grid");
         return SJ_ERR_INVALID;
@@ -395,18 +441,21 @@ SJ_API sj_status sj_batch_estimate(sj_grid g, uint64_t stride, uint64_t *sampleC
     if (stride == 0) stride = 100;
     uint64_t cnt = 0;
     SJ_TRY(batch_sample_count(g, stride, &cnt));
-    // policy of GPU.cu:1029-1055 (3 streams, 40M buffer, alpha 0.05)
-    const uint64_t est = cnt * stride;
-    uint32_t buf = 40000000u;
-    const double alpha = 0.05;
-    if (est < (uint64_t)buf * 3u) buf = (uint32_t)((double)est * (1.0 + alpha * 2.0) / 3.0);
-    if (buf == 0) buf = 1;
-    uint64_t nb = (uint64_t)std::ceil(((1.0 + alpha) * (double)est) / (double)buf);
-    if (nb == 0) nb = 1;
-    if (sampleCount) *sampleCount = cnt;
-    if (estimated) *estimated = est;
-    if (numBatches) *numBatches = nb;
-    if (bufferSize) *bufferSize = (uint64_t)(uint32_t)(1.5 * (double)buf);
+    batch_policy(cnt, stride, sampleCount, estimated, estimatedWithAlpha, numBatches, bufferSize);
+    return SJ_OK;
+}
+
+SJ_API sj_status sj_batch_estimate_workqueue(sj_grid g, int32_t lid, uint64_t stride, uint64_t *sampleCount,
+                                             uint64_t *estimated, uint64_t *estimatedWithAlpha,
+                                             uint64_t *numBatches, uint64_t *bufferSize) {
+    if (!g) {
+        set_error("sj_batch_estimate_workqueue: null grid");
+        return SJ_ERR_INVALID;
+    }
+    if (stride == 0) stride = 100;
+    uint64_t cnt = 0;
+    SJ_TRY(batch_sample_count_wq(g, lid ? 1 : 0, stride, &cnt));
+    batch_policy(cnt, stride, sampleCount, estimated, estimatedWithAlpha, numBatches, bufferSize);
     return SJ_OK;
 }
 

@@ -1521,7 +1521,9 @@ static sj_status build_tile_queue(sj_grid_s *g, const JoinArgs &a, uint64_t len,
     return SJ_OK;
 }
 
-sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
+// Workload per cell (w) and the cells by descending workload, ties by
+// ascending cell index (order), both on the device.
+static sj_status workload_order_device(sj_grid_s *g, int lid, DBuf<uint32_t> &w, DBuf<uint32_t> &order) {
     if (g->k > DEV_MAXK) {
         set_error("workload: k=%d above the device stencil limit %d", g->k, DEV_MAXK);
         return SJ_ERR_UNSUPPORTED;
@@ -1529,22 +1531,30 @@ sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t nne = g->nNonEmpty;
-    DBuf<uint32_t> w, key, key2, val, val2;
+    DBuf<uint32_t> key, key2, val2;
     SJ_TRY(cell_workload(g, lid, w));
     SJ_TRY(key.alloc(pool, nne));
     SJ_TRY(key2.alloc(pool, nne));
-    SJ_TRY(val.alloc(pool, nne));
+    SJ_TRY(order.alloc(pool, nne));
     SJ_TRY(val2.alloc(pool, nne));
-    hipLaunchKernelGGL(k_desc_keys, dim3(nblk(nne)), dim3(JB), 0, s, w.p, nne, key.p, val.p);
+    hipLaunchKernelGGL(k_desc_keys, dim3(nblk(nne)), dim3(JB), 0, s, w.p, nne, key.p, order.p);
     SJ_LAUNCH_CHECK();
-    uint32_t *kp = key.p, *ka = key2.p, *vp = val.p, *va = val2.p;
+    uint32_t *kp = key.p, *ka = key2.p, *vp = order.p, *va = val2.p;
     SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, nne, 32));
+    if (vp != order.p) std::swap(order, val2);
+    return SJ_OK;
+}
+
+sj_status grid_workload(sj_grid_s *g, int lid, int64_t *nb, uint32_t *order) {
+    const uint64_t nne = g->nNonEmpty;
+    DBuf<uint32_t> w, ord;
+    SJ_TRY(workload_order_device(g, lid, w, ord));
     std::vector<uint32_t> hw(nne);
-    SJ_TRY_HIP(hipStreamSynchronize(s));
+    SJ_TRY_HIP(hipStreamSynchronize(g->ctx->stream));
     SJ_TRY_HIP(hipMemcpy(hw.data(), w.p, nne * 4, hipMemcpyDeviceToHost));
     if (nb)
         for (uint64_t i = 0; i < nne; i++) nb[i] = hw[i];
-    if (order) SJ_TRY_HIP(hipMemcpy(order, vp, nne * 4, hipMemcpyDeviceToHost));
+    if (order) SJ_TRY_HIP(hipMemcpy(order, ord.p, nne * 4, hipMemcpyDeviceToHost));
     return SJ_OK;
 }
 
@@ -1574,9 +1584,13 @@ static bool record_walk(const sj_grid_s *g, const JoinArgs &a) {
     return a.useTiles && ((g->d == g->k && g->d <= DEV_MAXK) || a.mDP > 0);
 }
 
+sj_status tpp_point_order(sj_grid_s *g, const uint32_t *corder, DBuf<uint32_t> &order);
+sj_status launch_join_tpp(const JoinArgs &a, int k, int pass, uint64_t n, hipStream_t s);
+
 // replay: the fill replays the count pass's hit record (record_walk grids)
 static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec, bool replay = false) {
     hipStream_t s = g->ctx->stream;
+    if (a.tppOrder) return launch_join_tpp(a, g->k, pass, g->n, s);
     if (prec == SJ_PREC_FP32_RECHECK && a.mDP > 0) return launch_join_mfma(a, pass, g->ctx->cu_count, s, replay);
     if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
         return launch_join_wide(a, pass, g->ctx->cu_count, s);
@@ -1636,6 +1650,60 @@ sj_status launch_tile_cost(const JoinArgs &a, int k, uint32_t bound, uint32_t *k
     return SJ_OK;
 }
 
+// Sizes of the cells in workload order (the scan input of k_order_sample).
+__global__ void k_order_sizes(const uint32_t *order, const uint32_t *cellStart, uint64_t nne, uint32_t *size) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (i >= nne) return;
+    const uint32_t c = order[i];
+    size[i] = cellStart[c + 1] - cellStart[c];
+}
+
+// The first nsamp points of the cells taken in workload order (the points
+// of a cell in ascending ID): list[r] = sorted position of the r-th point.
+__global__ void k_order_sample(const uint32_t *order, const uint32_t *cellStart, const uint64_t *rankBase,
+                               uint64_t nne, uint64_t nsamp, uint32_t *list) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (i >= nne) return;
+    const uint64_t r0 = rankBase[i];
+    if (r0 >= nsamp) return;
+    const uint32_t c = order[i];
+    const uint32_t p0 = cellStart[c], p1 = cellStart[c + 1];
+    for (uint32_t p = p0; p < p1 && r0 + (p - p0) < nsamp; p++) list[r0 + (p - p0)] = p;
+}
+
+static sj_status sample_count_list(sj_grid_s *g, const uint32_t *list, uint64_t nsamp, uint64_t *count);
+
+// Work-queue estimate (callGPUBatchEstWorkQueue, GPU.cu:1072-1284;
+// kernelNDGridIndexWorkQueueBatchEstimatorOLD, kernel.cu:785-942): the
+// first n/stride points of the workload-ordered point list (full or LID
+// workload, as modes 9 and 10 order it, GPU.cu:1685-1692), full stencil.
+sj_status batch_sample_count_wq(sj_grid_s *g, int lid, uint64_t stride, uint64_t *count) {
+    if (g->k > DEV_MAXK) {
+        set_error("estimator: k=%d above the device stencil limit %d", g->k, DEV_MAXK);
+        return SJ_ERR_UNSUPPORTED;
+    }
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    const uint64_t n = g->n, nne = g->nNonEmpty;
+    if (stride == 0) stride = 1;
+    const uint64_t nsamp = (uint64_t)((double)n * (1.0 / (double)stride));
+    *count = 0;
+    if (nsamp == 0) return SJ_OK;
+    DBuf<uint32_t> w, order, size, list;
+    DBuf<uint64_t> rank;
+    SJ_TRY(workload_order_device(g, lid, w, order));
+    SJ_TRY(size.alloc(pool, nne));
+    SJ_TRY(rank.alloc(pool, nne + 1));
+    SJ_TRY(list.alloc(pool, nsamp));
+    hipLaunchKernelGGL(k_order_sizes, dim3(nblk(nne)), dim3(JB), 0, s, order.p, g->cellStart.p, nne, size.p);
+    SJ_LAUNCH_CHECK();
+    SJ_TRY(scan_exclusive_u32_to_u64(pool, s, size.p, nne, rank.p));
+    hipLaunchKernelGGL(k_order_sample, dim3(nblk(nne)), dim3(JB), 0, s, order.p, g->cellStart.p, rank.p, nne, nsamp,
+                       list.p);
+    SJ_LAUNCH_CHECK();
+    return sample_count_list(g, list.p, nsamp, count);
+}
+
 // Sampled estimate of |S| (every stride-th original point, full stencil).
 sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
     Pool *pool = &g->ctx->pool;
@@ -1649,26 +1717,34 @@ sj_status batch_sample_count(sj_grid_s *g, uint64_t stride, uint64_t *count) {
     const uint64_t nsamp = (uint64_t)((double)n * (1.0 / (double)stride));
     *count = 0;
     if (nsamp == 0) return SJ_OK;
-    DBuf<uint32_t> posOf, list, own;
-    DBuf<unsigned long long> stat;
+    DBuf<uint32_t> posOf, list;
     SJ_TRY(posOf.alloc(pool, n));
     SJ_TRY(list.alloc(pool, nsamp));
-    SJ_TRY(own.alloc(pool, n));
-    SJ_TRY(stat.alloc(pool, 8));
     hipLaunchKernelGGL(k_sample_positions, dim3(nblk(n)), dim3(JB), 0, s, g->sortedIds.p, n, stride, nsamp,
                        posOf.p, list.p, 0);
     SJ_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sample_positions, dim3(nblk(nsamp)), dim3(JB), 0, s, g->sortedIds.p, n, stride,
                        nsamp, posOf.p, list.p, 1);
     SJ_LAUNCH_CHECK();
+    return sample_count_list(g, list.p, nsamp, count);
+}
+
+// Full-stencil hits of the sorted positions list[0..nsamp), summed.
+static sj_status sample_count_list(sj_grid_s *g, const uint32_t *list, uint64_t nsamp, uint64_t *count) {
+    Pool *pool = &g->ctx->pool;
+    hipStream_t s = g->ctx->stream;
+    DBuf<uint32_t> own;
+    DBuf<unsigned long long> stat;
+    SJ_TRY(own.alloc(pool, g->n));
+    SJ_TRY(stat.alloc(pool, 8));
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p), 8, 0ull));
     JoinArgs a = base_args(g, SJ_FORM_FULL, eps_threshold(g->eps));
     a.ownCnt = own.p;
     a.mirCnt = own.p;  // unused by FULL
     a.stat = stat.p;
-    hipLaunchKernelGGL(pick_sample(g->k, g->d), dim3(nblk(nsamp)), dim3(JB), 0, s, a, list.p, nsamp);
+    hipLaunchKernelGGL(pick_sample(g->k, g->d), dim3(nblk(nsamp)), dim3(JB), 0, s, a, list, nsamp);
     SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sum_gather, dim3(nblk(nsamp)), dim3(JB), 0, s, own.p, list.p, nsamp, stat.p + 5);
+    hipLaunchKernelGGL(k_sum_gather, dim3(nblk(nsamp)), dim3(JB), 0, s, own.p, list, nsamp, stat.p + 5);
     SJ_LAUNCH_CHECK();
     uint64_t *pin = g->ctx->pinned;
     SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + 5, 8, hipMemcpyDeviceToHost, s));
@@ -1746,9 +1822,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         return SJ_ERR_INVALID;
     }
     if (P->form < 0 || P->form > 3 || P->precision < 0 || P->precision > 1 || P->schedule < 0 ||
-        P->schedule > 1) {
+        P->schedule > 2) {
         set_error("join: invalid form/precision/schedule");
         return SJ_ERR_INVALID;
+    }
+    if (P->schedule == SJ_SCHED_THREADS_PER_POINT &&
+        (P->form != SJ_FORM_FULL || P->precision != SJ_PREC_FP64)) {
+        set_error("join: the threads-per-point schedule runs the full stencil in FP64 (reference mode 8)");
+        return SJ_ERR_UNSUPPORTED;
     }
     if (P->block_size != 0 && P->block_size != JB) {
         set_error("join: block_size must be 0 or %d", JB);
@@ -1773,6 +1854,17 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     // precision form: FP32 copy relative to the grid origin + margin
     int prec = P->precision;
     if (prec == SJ_PREC_FP32_RECHECK) SJ_TRY(prepare_fp32(g, a, &prec));
+
+    // threads per point: the points in workload order, no tiles
+    DBuf<uint32_t> tppCellW, tppCellOrder, tppOrder;
+    if (P->schedule == SJ_SCHED_THREADS_PER_POINT) {
+        tw.start();
+        SJ_TRY(workload_order_device(g, 0, tppCellW, tppCellOrder));
+        SJ_TRY(tpp_point_order(g, tppCellOrder.p, tppOrder));
+        tw.stop();
+        a.tppOrder = tppOrder.p;
+        a.useTiles = 0;
+    }
 
     // row-aligned tiles; work queue of tiles in descending workload
     DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr;
@@ -1978,6 +2070,16 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         } filled[2], copied[2];
         DBuf<int32_t> dbuf[2];
         DBuf<uint32_t> bCount[2];
+        // declared after the buffers, so destroyed before them: every exit
+        // from here, error paths included, waits for the downloads in flight
+        // before the device buffers go back to the pool
+        struct CopyDrain {
+            hipStream_t s, cs;
+            ~CopyDrain() {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamSynchronize(cs);
+            }
+        } drain{s, cs};
         for (int j = 0; j < 2; j++) {
             SJ_TRY(dbuf[j].alloc(pool, P->capacity));
             SJ_TRY(bCount[j].alloc(pool, 1));

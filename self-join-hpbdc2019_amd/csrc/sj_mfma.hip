@@ -77,9 +77,10 @@ int mfma_padded_dims(int d) {
 //    thi = T/(1-rho), each widened by 4u for the FP32 compare;
 //  * 5% on alpha and beta covers their own FP32 evaluation and the
 //    rounding of s; DP * 2^-120 covers denormal flushing in the matrix core.
-void mfma_consts(JoinArgs &a, double T, double R) {
+MfmaBand mfma_band(double T, double R, int d) {
+    MfmaBand b;
     const double u = std::ldexp(1.0, -24);
-    const int DP = mfma_padded_dims(a.d);
+    const int DP = mfma_padded_dims(d);
     const double gam = 1.01 * DP * u;
     double alpha = (2.0 * gam + 3.0 * u), beta = 0.0;
     if (R >= 0.0) {
@@ -91,23 +92,34 @@ void mfma_consts(JoinArgs &a, double T, double R) {
     }
     alpha *= 1.05;
     beta = beta * 1.05 + DP * std::ldexp(1.0, -120);
-    const double rho = 1.01 * (a.d + 2) * std::ldexp(1.0, -53);
+    const double rho = 1.01 * (d + 2) * std::ldexp(1.0, -53);
     const double tlo = T / (1.0 + rho) * (1.0 - 4.0 * u);
     const double thi = T / (1.0 - rho) * (1.0 + 4.0 * u);
     float f = (float)alpha;
     if ((double)f < alpha) f = std::nextafter(f, INFINITY);
-    a.mAlpha = f;
+    b.alpha = f;
     f = (float)beta;
     if ((double)f < beta) f = std::nextafter(f, INFINITY);
-    a.mBeta = f;
+    b.beta = f;
     f = (float)tlo;
     if ((double)f > tlo) f = std::nextafter(f, -INFINITY);
-    a.mTlo = f;
+    b.tlo = f;
     f = (float)thi;
     if ((double)f < thi) f = std::nextafter(f, INFINITY);
-    a.mThi = f;
-    a.mT32 = (float)T;
-    a.mDP = DP;
+    b.thi = f;
+    b.t32 = (float)T;
+    b.DP = DP;
+    return b;
+}
+
+void mfma_consts(JoinArgs &a, double T, double R) {
+    const MfmaBand b = mfma_band(T, R, a.d);
+    a.mAlpha = b.alpha;
+    a.mBeta = b.beta;
+    a.mTlo = b.tlo;
+    a.mThi = b.thi;
+    a.mT32 = b.t32;
+    a.mDP = b.DP;
     a.mSrc32 = R >= 0.0 ? 1 : 0;
 }
 

@@ -40,8 +40,8 @@ Pool::~Pool() {
 }
 
 sj_status Pool::alloc(size_t bytes, void **out) {
-    size_t c = size_class(bytes);
     std::lock_guard<std::mutex> lk(mu_);
+    size_t c = size_class(bytes);
     auto it = free_.find(c);
     if (it != free_.end() && !it->second.empty()) {
         *out = it->second.back();

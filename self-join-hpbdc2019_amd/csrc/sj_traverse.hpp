@@ -42,6 +42,7 @@ struct JoinArgs {
     const uint32_t *nTilesDev;  // number of tiles (device)
     const uint32_t *tileOrder;  // work queue order of tiles, or null
     uint32_t *qctr;
+    const uint32_t *tppOrder;   // threads-per-point schedule: points in workload order, or null
     // rows written by this launch (sorted positions)
     uint32_t wBegin, wEnd;
     // count pass
@@ -384,6 +385,14 @@ constexpr int MFMA_MAX_D = 64;  // FP32+recheck joins with d <= MFMA_MAX_D run o
 // Padded dims of the matrix-core operands (4, 8, 16, 32 or 64), 0 above
 // MFMA_MAX_D.
 int mfma_padded_dims(int d);
+// Decision constants of the matrix-core walk: with s = |q|^2 + |c|^2 and
+// M = alpha s + beta, d2 + M <= tlo is a certain hit and d2 - M > thi a
+// certain miss of the canonical FP64 predicate (derivation in sj_mfma.hip).
+struct MfmaBand {
+    float alpha, beta, tlo, thi, t32;
+    int DP;
+};
+MfmaBand mfma_band(double T, double R, int d);
 // Fills the matrix-core fields of a (mAlpha .. mSrc32) for threshold T;
 // R >= 0: staging from the FP32 copy whose coordinates are within R of the
 // origin (Euclidean), R < 0: staging centred in FP64.

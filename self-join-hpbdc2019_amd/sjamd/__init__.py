@@ -19,6 +19,7 @@ Reference interface map (benoitgallet/self-join-hpbdc2019):
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -28,7 +29,7 @@ MAXK = 12
 
 FULL, UNICOMP, BUNICOMP, LID = 0, 1, 2, 3
 FP64, FP32_RECHECK = 0, 1
-STRIDED, WORKQUEUE = 0, 1
+STRIDED, WORKQUEUE, THREADS_PER_POINT = 0, 1, 2
 
 FORMS = {"full": FULL, "unicomp": UNICOMP, "bunicomp": BUNICOMP, "lid": LID}
 
@@ -39,6 +40,7 @@ SEARCH_MODES = {
     5: (BUNICOMP, STRIDED),
     6: (LID, STRIDED),
     7: (FULL, WORKQUEUE),
+    8: (FULL, THREADS_PER_POINT),
     9: (FULL, WORKQUEUE),
     10: (LID, WORKQUEUE),
 }
@@ -105,9 +107,10 @@ EXPORTS = [
     "sj_join", "sj_self_join", "sj_result_get_stats", "sj_result_total_pairs",
     "sj_result_num_points", "sj_result_device_table", "sj_result_device_csr", "sj_result_copy_csr", "sj_result_copy_pairs",
     "sj_result_neighbor_table", "sj_result_batch_data", "sj_result_free", "sj_workload",
-    "sj_batch_estimate", "sj_import_csv", "sj_sort_in_nd_bins", "sj_free_host", "sj_eps_threshold",
+    "sj_batch_estimate", "sj_batch_estimate_workqueue", "sj_import_csv", "sj_sort_in_nd_bins", "sj_free_host", "sj_eps_threshold",
     "sj_many_create", "sj_many_run", "sj_many_autotune", "sj_many_set_tuning", "sj_many_get_tuning",
     "sj_many_instance", "sj_many_stats", "sj_many_times", "sj_many_count", "sj_many_free",
+    "sj_fp32_band", "sj_mfma_band",
 ]
 
 _lib = None
@@ -126,6 +129,11 @@ def lib():
     L.sj_last_error.restype = C.c_char_p
     L.sj_eps_threshold.restype = C.c_double
     L.sj_eps_threshold.argtypes = [C.c_double]
+    fp = C.POINTER(C.c_float)
+    L.sj_fp32_band.restype = i32
+    L.sj_fp32_band.argtypes = [C.c_double, C.c_double, i32, fp, fp, fp]
+    L.sj_mfma_band.restype = i32
+    L.sj_mfma_band.argtypes = [C.c_double, C.c_double, i32, fp, fp, fp, fp]
     L.sj_context_create.argtypes = [i32, vp, P(vp)]
     L.sj_context_destroy.argtypes = [vp]
     L.sj_context_destroy.restype = None
@@ -152,7 +160,8 @@ def lib():
     L.sj_result_free.argtypes = [vp]
     L.sj_result_free.restype = None
     L.sj_workload.argtypes = [vp, i32, vp, vp]
-    L.sj_batch_estimate.argtypes = [vp, u64, P(u64), P(u64), P(u64), P(u64)]
+    L.sj_batch_estimate.argtypes = [vp, u64, P(u64), P(u64), P(u64), P(u64), P(u64)]
+    L.sj_batch_estimate_workqueue.argtypes = [vp, i32, u64, P(u64), P(u64), P(u64), P(u64), P(u64)]
     L.sj_import_csv.argtypes = [C.c_char_p, i32, P(P(C.c_double)), P(u64)]
     L.sj_sort_in_nd_bins.argtypes = [P(C.c_double), u64, i32, i32]
     L.sj_free_host.argtypes = [vp]
@@ -185,6 +194,20 @@ def _np_ptr(a):
 
 def eps_threshold(eps):
     return lib().sj_eps_threshold(float(eps))
+
+
+def fp32_band(T, R, D):
+    """(usable, lo, hi, t32) of the VALU FP32 precision form."""
+    lo, hi, t32 = C.c_float(), C.c_float(), C.c_float()
+    u = lib().sj_fp32_band(float(T), float(R), int(D), C.byref(lo), C.byref(hi), C.byref(t32))
+    return bool(u), lo.value, hi.value, t32.value
+
+
+def mfma_band(T, R, d):
+    """(DP, alpha, beta, tlo, thi) of the matrix-core FP32 form."""
+    v = [C.c_float() for _ in range(4)]
+    dp = lib().sj_mfma_band(float(T), float(R), int(d), *[C.byref(x) for x in v])
+    return (dp,) + tuple(x.value for x in v)
 
 
 _hip_rt = None
@@ -252,9 +275,17 @@ class Context:
         sp = C.c_void_p(stream.cuda_stream) if stream is not None else C.c_void_p()
         _check(lib().sj_context_create(device, sp, C.byref(h)))
         self.h = h
+        # grids, results and batches made on this context: freed before it
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
+        return child
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.free()
             lib().sj_context_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -265,13 +296,21 @@ class Context:
             pass
 
     def to_device(self, points):
-        """float64 (n, d) array -> contiguous device tensor."""
+        """float64 (n, d) array -> contiguous device tensor.  A conversion or
+        copy runs on torch's current stream; when that is not the context's
+        stream it is waited for here, so the library never reads the
+        coordinates before they are written."""
         torch = self.torch
         if isinstance(points, torch.Tensor):
             t = points
         else:
             t = torch.from_numpy(np.ascontiguousarray(points, dtype=np.float64))
-        return t.to(device=f"cuda:{self.device}", dtype=torch.float64).contiguous()
+        out = t.to(device=f"cuda:{self.device}", dtype=torch.float64).contiguous()
+        if out is not t or out.data_ptr() != t.data_ptr():
+            cur = torch.cuda.current_stream(self.device)
+            if self.stream is None or cur.cuda_stream != self.stream.cuda_stream:
+                cur.synchronize()
+        return out
 
     def grid_build(self, points, k, eps):
         t = self.to_device(points)
@@ -279,7 +318,7 @@ class Context:
         h = C.c_void_p()
         _check(lib().sj_grid_build(self.h, C.c_void_p(t.data_ptr()), n, d, k, float(eps),
                                    C.byref(h)))
-        return Grid(self, h, t)
+        return self._adopt(Grid(self, h, t))
 
     def self_join(self, points, k, eps, **kw):
         t = self.to_device(points)
@@ -288,11 +327,11 @@ class Context:
         h = C.c_void_p()
         _check(lib().sj_self_join(self.h, C.c_void_p(t.data_ptr()), n, d, k, float(eps),
                                   C.byref(p), C.byref(h)))
-        return Result(self, h)
+        return self._adopt(Result(self, h))
 
     def many(self, instances, **kw):
         """Plan many joins; instances: iterable of (points, k, eps)."""
-        return Many(self, instances, **kw)
+        return self._adopt(Many(self, instances, **kw))
 
 
 class Grid:
@@ -345,7 +384,7 @@ class Grid:
         p = make_params(**kw)
         h = C.c_void_p()
         _check(lib().sj_join(self.h, C.byref(p), C.byref(h)))
-        return Result(self.ctx, h)
+        return self.ctx._adopt(Result(self.ctx, h))
 
     def workload(self, lid=False):
         w = np.zeros(self.nNonEmpty, dtype=np.int64)
@@ -353,11 +392,19 @@ class Grid:
         _check(lib().sj_workload(self.h, 1 if lid else 0, _np_ptr(w), _np_ptr(o)))
         return w, o
 
-    def batch_estimate(self, stride=100):
-        vals = [C.c_uint64() for _ in range(4)]
-        _check(lib().sj_batch_estimate(self.h, stride, *[C.byref(v) for v in vals]))
-        cnt, est, nb, buf = (v.value for v in vals)
-        return dict(sample_count=cnt, estimated=est, num_batches=nb, buffer_size=buf)
+    def batch_estimate(self, stride=100, workqueue=None):
+        """callGPUBatchEst (workqueue=None: every stride-th point) or
+        callGPUBatchEstWorkQueue (workqueue='full' or 'lid': the first
+        n/stride points in that workload order)."""
+        vals = [C.c_uint64() for _ in range(5)]
+        if workqueue is None:
+            _check(lib().sj_batch_estimate(self.h, stride, *[C.byref(v) for v in vals]))
+        else:
+            lid = {"full": 0, "lid": 1}[workqueue]
+            _check(lib().sj_batch_estimate_workqueue(self.h, lid, stride, *[C.byref(v) for v in vals]))
+        cnt, est, esta, nb, buf = (v.value for v in vals)
+        return dict(sample_count=cnt, estimated=est, estimated_with_alpha=esta, num_batches=nb,
+                    buffer_size=buf)
 
 
 class Result:

@@ -296,6 +296,33 @@ def test_sample_count_matches_brute():
     assert g.sample_count(100) == int(cnt[::100][:30].sum())
 
 
+@pytest.mark.parametrize("lid", [False, True])
+def test_sample_count_workqueue_matches_brute(lid):
+    # the sample is the first n/stride points of the workload-ordered list;
+    # the order comes from a numpy count of each cell's 3x3 stencil and a
+    # stable sort, the hits from brute force over all points
+    pts = D.exponential_clipped(4_000, 2, 43, 40.0)
+    eps = 0.01
+    g = O.OracleGrid(pts, 2, eps)
+    coords = g.cellCoord.astype(np.int64)
+    cells = {tuple(c.tolist()): i for i, c in enumerate(coords)}
+    size = (g.indexMax - g.indexMin + 1).astype(np.int64)
+    w = np.zeros(g.nNonEmpty, dtype=np.int64)
+    for i, c in enumerate(coords.tolist()):
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                j = cells.get((c[0] + dx, c[1] + dy))
+                if j is not None and (not lid or g.cellLinear[j] >= g.cellLinear[i]):
+                    w[i] += size[j]
+    order = np.argsort(-w, kind="stable")
+    plist = np.concatenate([g.indexLookup[g.indexMin[c]:g.indexMax[c] + 1] for c in order])
+    off, _ = O.brute(pts, eps)
+    cnt = np.diff(off.astype(np.int64))
+    for stride in (100, 37):
+        nsamp = int(pts.shape[0] * (1.0 / stride))
+        assert g.sample_count_workqueue(stride, lid) == int(cnt[plist[:nsamp]].sum())
+
+
 def test_neighbor_table_run_length():
     keys = np.array([0, 0, 2, 2, 2, 3], dtype=np.int32)
     vals = np.array([0, 1, 2, 3, 4, 3], dtype=np.int32)
