@@ -265,6 +265,34 @@ SJ_API sj_status sj_many_times(sj_many m, double *ms_plan, double *ms_grid, doub
 SJ_API int32_t sj_many_count(sj_many m);
 SJ_API void sj_many_free(sj_many m);
 
+/* ---- Allocation-free self-join -----------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): sj_self_join with every device buffer of the grid
+ * build and the join taken from one static device array of
+ * SJ_STATIC_SCRATCH_MIB MiB per device (held from the library's load on
+ * that device until the process ends), and the outputs written to the
+ * caller's arrays: offsets[n+1] (uint64) and rowIds[n] (rows in cell order,
+ * as sj_result_device_table), ids[ids_capacity] (int32).  *total receives
+ * |S|; above ids_capacity the call returns SJ_ERR_OVERFLOW with *total set,
+ * and nothing is written to ids.  Calls no hipMalloc, hipFree or other
+ * allocating routine; the context must exist (sj_context_create looks up
+ * the array's device address).  Inputs whose scratch bound
+ * (sj_static_need) exceeds the array, and capacity > 0 in params, are
+ * rejected with SJ_ERR_OVERFLOW / SJ_ERR_UNSUPPORTED before any device
+ * work.  The fill evaluates distances again (no hit record).  Every call
+ * on a device shares that device's array: calls that overlap on one device,
+ * from any context or thread, race. */
+#define SJ_STATIC_SCRATCH_MIB 256
+SJ_API sj_status sj_self_join_static(sj_context ctx, const double *d_points, uint64_t n, int32_t d,
+                                     int32_t k, double eps, const sj_join_params *params,
+                                     uint64_t *d_offsets, int32_t *d_ids, uint64_t ids_capacity,
+                                     uint32_t *d_rowIds, uint64_t *total);
+/* Bytes of static scratch sj_self_join_static may need for n points (an
+ * upper bound), and the size of the array. */
+SJ_API uint64_t sj_static_need(uint64_t n, int32_t d, int32_t k, const sj_join_params *params);
+SJ_API uint64_t sj_static_scratch_bytes(void);
+
 /* Eps threshold T = max{x : sqrt_rn(x) <= eps}; sqrt(d2) <= eps iff d2 <= T. */
 SJ_API double sj_eps_threshold(double eps);
 

@@ -21,6 +21,7 @@ double eps_threshold(double eps);
 sj_status import_csv(const char *path, int d, double **pts, uint64_t *n);
 sj_status sort_in_nd_bins(double *pts, uint64_t n, int d, int stable);
 sj_status result_to_orig(sj_result_s *r);
+void static_scratch_init(int device);
 }  // namespace sj
 
 using namespace sj;
@@ -110,6 +111,9 @@ SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out
         return SJ_ERR_INVALID;
     }
     SJ_TRY_HIP(hipSetDevice(device));
+    // the device address of this device's allocation-free scratch, looked up
+    // once per device with the device current
+    static_scratch_init(device);
     sj_context c = new (std::nothrow) sj_context_s();
     if (!c) return SJ_ERR_NOMEM;
     c->device = device;

@@ -48,6 +48,11 @@ void set_error(const char *fmt, ...);
 // Caching allocator: power-of-two size classes, freed blocks are kept for
 // reuse so steady-state joins make no hipMalloc calls.  A pinned pool
 // hands out page-locked host memory (hipHostMalloc) instead.
+//
+// Arena mode (arena_begin .. arena_end) serves every allocation from one
+// caller-given device range instead, in order and 256-byte aligned, with a
+// fixed bookkeeping table: no hipMalloc and no host allocation.  A release
+// frees the topmost block and any released blocks under it.
 class Pool {
   public:
     explicit Pool(bool pinned = false) : pinned_(pinned) {}
@@ -55,12 +60,30 @@ class Pool {
     sj_status alloc(size_t bytes, void **out);
     void release(void *p);
     void trim();
+    void arena_begin(void *base, size_t bytes);
+    void arena_end();
+    bool arena_active() const { return abase_ != nullptr; }
+    size_t arena_peak() const { return apeak_; }
+    // bytes left above the topmost block (0 outside arena mode)
+    size_t arena_free() const {
+        const size_t off = (atop_ + 255) & ~(size_t)255;
+        return abase_ && off < acap_ ? acap_ - off : 0;
+    }
 
   private:
     bool pinned_;
     std::mutex mu_;
     std::map<size_t, std::vector<void *>> free_;
     std::map<void *, size_t> live_;
+    static constexpr int ARENA_BLOCKS = 256;
+    struct ArenaBlock {
+        size_t off;
+        bool live;
+    };
+    char *abase_ = nullptr;
+    size_t acap_ = 0, atop_ = 0, apeak_ = 0;
+    ArenaBlock ablk_[ARENA_BLOCKS];
+    int an_ = 0;
 };
 
 // RAII device buffer drawn from a Pool.
@@ -174,6 +197,9 @@ struct sj_grid_s {
     sj::DBuf<float> sortedPts32;     // lazily built for the FP32 form
     double ms_build = 0.0;
     bool transient = false;          // made for one join, which may take its buffers
+    // arena mode: device bytes the join still needs after the grid; the
+    // dense lookup table is built only if it fits beside them
+    uint64_t reserveAfter = 0;
 };
 
 struct sj_batch_host {
@@ -184,6 +210,12 @@ struct sj_batch_host {
 
 struct sj_result_s {
     sj_context ctx = nullptr;
+    // caller-owned outputs (the allocation-free entry point): offsets[n+1],
+    // ids[extIdsCap], rowIds[n]; null: the result allocates its own
+    uint64_t *extOffsets = nullptr;
+    int32_t *extIds = nullptr;
+    uint32_t *extRowIds = nullptr;
+    uint64_t extIdsCap = 0;
     uint64_t n = 0;
     uint64_t total = 0;
     bool on_device = false;

@@ -396,6 +396,8 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     // posLB gives both a cell's sorted range and whether it is non-empty
     const uint64_t tc = g->gp.totalCells;
     g->dense = tc <= (1ull << 26) || tc <= 16ull * n;
+    if (g->dense && pool->arena_active() && (tc + 1) * 4 + 256 + g->reserveAfter > pool->arena_free())
+        g->dense = false;  // a static scratch too small for the table: binary search
     if (g->dense) {
         SJ_TRY(g->posLB.alloc(pool, tc + 1));
         hipLaunchKernelGGL(k_pos_lb, dim3(blocks_for(tc / POSLB_RUN + 1)), dim3(GB), 0, s, g->cellLinear.p,

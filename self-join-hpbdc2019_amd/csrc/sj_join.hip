@@ -1926,9 +1926,15 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();
 
-    // row offsets in cell order
+    // row offsets in cell order (the caller's array when it gave one)
     ts.start();
-    SJ_TRY(r->d_offsets.alloc(pool, n + 1));
+    if (r->extOffsets) {
+        r->d_offsets.p = r->extOffsets;
+        r->d_offsets.n = n + 1;
+        r->d_offsets.pool = nullptr;
+    } else {
+        SJ_TRY(r->d_offsets.alloc(pool, n + 1));
+    }
     SJ_TRY(scan_exclusive_sum2_u32_to_u64(pool, s, own.p, mir.p, n, r->d_offsets.p));
     ts.stop();
     // |S|, the statistics and the record's state in one readback
@@ -1974,8 +1980,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     }
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
     // row IDs: a grid made for this call alone hands its sorted IDs over at
-    // the end; otherwise they are copied
-    if (!g->transient) {
+    // the end; otherwise they are copied (into the caller's array when it
+    // gave one)
+    if (r->extRowIds) {
+        SJ_TRY_HIP(hipMemcpyAsync(r->extRowIds, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
+        r->d_rowIds.p = r->extRowIds;
+        r->d_rowIds.n = n;
+        r->d_rowIds.pool = nullptr;
+    } else if (!g->transient) {
         SJ_TRY(r->d_rowIds.alloc(pool, n));
         SJ_TRY_HIP(hipMemcpyAsync(r->d_rowIds.p, g->sortedIds.p, n * 4, hipMemcpyDeviceToDevice, s));
     }
@@ -1988,7 +2000,18 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     const bool whole = P->capacity == 0 || total <= P->capacity;
     double ms_fill = 0.0;
     if (whole) {
-        SJ_TRY(r->d_ids.alloc(pool, total));
+        if (r->extIds) {
+            if (total > r->extIdsCap) {
+                set_error("join: %llu result pairs exceed the caller's capacity of %llu",
+                          (unsigned long long)total, (unsigned long long)r->extIdsCap);
+                return SJ_ERR_OVERFLOW;
+            }
+            r->d_ids.p = r->extIds;
+            r->d_ids.n = total;
+            r->d_ids.pool = nullptr;
+        } else {
+            SJ_TRY(r->d_ids.alloc(pool, total));
+        }
         // the replayed LDS tile walk places mirrored hits at the recorded
         // reservations; every other fill takes them from the cursors
         if (!(replay && a.mDP == 0)) {
@@ -2155,7 +2178,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         r->stats.num_batches = win.size();
         r->stats.fill_from_record = replay ? 1 : 0;
     }
-    if (g->transient) r->d_rowIds = std::move(g->sortedIds);
+    if (g->transient && !r->extRowIds) r->d_rowIds = std::move(g->sortedIds);
     r->stats.ms_workload = tw.ms();
     r->stats.ms_count = tc.ms();
     r->stats.ms_scan = ts.ms();

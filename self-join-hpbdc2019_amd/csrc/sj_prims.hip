@@ -39,8 +39,37 @@ Pool::~Pool() {
     for (auto &kv : live_) pool_free(pinned_, kv.first);
 }
 
+void Pool::arena_begin(void *base, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    abase_ = static_cast<char *>(base);
+    acap_ = bytes;
+    atop_ = apeak_ = 0;
+    an_ = 0;
+}
+
+void Pool::arena_end() {
+    std::lock_guard<std::mutex> lk(mu_);
+    abase_ = nullptr;
+    acap_ = atop_ = 0;
+    an_ = 0;
+}
+
 sj_status Pool::alloc(size_t bytes, void **out) {
     std::lock_guard<std::mutex> lk(mu_);
+    if (abase_) {
+        const size_t off = (atop_ + 255) & ~(size_t)255;
+        const size_t need = bytes ? bytes : 1;
+        if (an_ == ARENA_BLOCKS || off > acap_ || need > acap_ - off) {
+            set_error("static scratch exhausted: %zu bytes requested at offset %zu of %zu (%d blocks)", need, off,
+                      acap_, an_);
+            return SJ_ERR_NOMEM;
+        }
+        ablk_[an_++] = {off, true};
+        atop_ = off + need;
+        apeak_ = std::max(apeak_, atop_);
+        *out = abase_ + off;
+        return SJ_OK;
+    }
     size_t c = size_class(bytes);
     auto it = free_.find(c);
     if (it != free_.end() && !it->second.empty()) {
@@ -72,6 +101,17 @@ sj_status Pool::alloc(size_t bytes, void **out) {
 
 void Pool::release(void *p) {
     std::lock_guard<std::mutex> lk(mu_);
+    char *q = static_cast<char *>(p);
+    if (abase_ && q >= abase_ && q < abase_ + acap_) {
+        const size_t off = (size_t)(q - abase_);
+        for (int i = an_ - 1; i >= 0; i--)
+            if (ablk_[i].live && ablk_[i].off == off) {
+                ablk_[i].live = false;
+                break;
+            }
+        while (an_ > 0 && !ablk_[an_ - 1].live) atop_ = ablk_[--an_].off;
+        return;
+    }
     auto it = live_.find(p);
     if (it == live_.end()) return;
     free_[it->second].push_back(p);

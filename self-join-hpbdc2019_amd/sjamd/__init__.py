@@ -111,6 +111,7 @@ EXPORTS = [
     "sj_many_create", "sj_many_run", "sj_many_autotune", "sj_many_set_tuning", "sj_many_get_tuning",
     "sj_many_instance", "sj_many_stats", "sj_many_times", "sj_many_count", "sj_many_free",
     "sj_fp32_band", "sj_mfma_band",
+    "sj_self_join_static", "sj_static_need", "sj_static_scratch_bytes",
 ]
 
 _lib = None
@@ -178,6 +179,11 @@ def lib():
     L.sj_many_count.restype = i32
     L.sj_many_free.argtypes = [vp]
     L.sj_many_free.restype = None
+    L.sj_self_join_static.argtypes = [vp, vp, u64, i32, i32, C.c_double, P(JoinParams), vp, vp, u64, vp, P(u64)]
+    L.sj_static_need.argtypes = [u64, i32, i32, P(JoinParams)]
+    L.sj_static_need.restype = u64
+    L.sj_static_scratch_bytes.argtypes = []
+    L.sj_static_scratch_bytes.restype = u64
     _lib = L
     return L
 
