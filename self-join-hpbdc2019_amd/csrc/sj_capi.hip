@@ -139,6 +139,14 @@ SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out
         set_error("hipHostMalloc: %s", hipGetErrorString(e));
         return SJ_ERR_NOMEM;
     }
+    e = hipEventCreateWithFlags(&c->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipHostFree(c->pinned);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        set_error("hipEventCreate: %s", hipGetErrorString(e));
+        return SJ_ERR_HIP;
+    }
     c->cu_count = device_cu_count(device);
     *out = c;
     return SJ_OK;
@@ -164,6 +172,7 @@ SJ_API void sj_context_destroy(sj_context ctx) {
         (void)hipStreamDestroy(ctx->copyStream);
     }
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->ev) (void)hipEventDestroy(ctx->ev);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

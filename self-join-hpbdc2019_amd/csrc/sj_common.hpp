@@ -172,10 +172,14 @@ struct sj_context_s {
     sj::Pool hpool{true};        // pinned host memory: result batches, staging
     hipStream_t copyStream = nullptr;  // result-batch downloads (created on first use)
     uint64_t *pinned = nullptr;  // 64 scalars of pinned readback space
+    hipEvent_t ev = nullptr;     // marks a readback into `pinned` the host waits for
     int cu_count = 256;
     // hit-record space of the joins on this context (directory entries,
     // mask words): grows to the demand a join reports when it did not fit
     uint64_t recDirCap = 0, recMaskCap = 0;
+    // neighbour IDs of the largest result seen (plus a margin): a join whose
+    // result fits fills without first reading |S| back to the host
+    uint64_t idsHint = 0;
 };
 
 struct sj_grid_s {

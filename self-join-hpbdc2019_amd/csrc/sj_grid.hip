@@ -19,27 +19,52 @@ namespace sj {
 
 constexpr int GB = 256;
 
+// Per-block min/max of the first k coordinates and a check that all d are
+// finite.  DT = d when d <= 8 (the row is loaded in full, four rows in
+// flight per thread), DT = 0 for any d.
+template <int DT>
 __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, int d, int k,
                                                double *partial, uint32_t *flags) {
-    double mn[SJ_MAXK], mx[SJ_MAXK];
+    constexpr int KM = DT > 0 ? (DT < SJ_MAXK ? DT : SJ_MAXK) : SJ_MAXK;
+    constexpr int U = DT > 0 ? 4 : 1;
+    double mn[KM], mx[KM];
 #pragma unroll
-    for (int j = 0; j < SJ_MAXK; j++) {
+    for (int j = 0; j < KM; j++) {
         mn[j] = INFINITY;
         mx[j] = -INFINITY;
     }
     bool bad = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * GB) {
-        const double *p = pts + i * (uint64_t)d;
-        for (int j = 0; j < d; j++) {
-            double x = p[j];
-            bad |= !isfinite(x);
-        }
+    const uint64_t stride = (uint64_t)gridDim.x * GB;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * GB + threadIdx.x; i0 < n; i0 += stride * U) {
+        if constexpr (DT > 0) {
+            double v[U][DT];
 #pragma unroll
-        for (int j = 0; j < SJ_MAXK; j++) {
-            if (j < k) {
-                double x = p[j];
-                mn[j] = fmin(mn[j], x);
-                mx[j] = fmax(mx[j], x);
+            for (int u = 0; u < U; u++) {
+                const uint64_t i = i0 + (uint64_t)u * stride;
+#pragma unroll
+                for (int j = 0; j < DT; j++) v[u][j] = i < n ? pts[i * DT + j] : pts[i0 * DT + j];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+#pragma unroll
+                for (int j = 0; j < DT; j++) {
+                    bad |= !isfinite(v[u][j]);
+                    if (j < k) {
+                        mn[j] = fmin(mn[j], v[u][j]);
+                        mx[j] = fmax(mx[j], v[u][j]);
+                    }
+                }
+            }
+        } else {
+            const double *p = pts + i0 * (uint64_t)d;
+            for (int j = 0; j < d; j++) bad |= !isfinite(p[j]);
+#pragma unroll
+            for (int j = 0; j < KM; j++) {
+                if (j < k) {
+                    const double x = p[j];
+                    mn[j] = fmin(mn[j], x);
+                    mx[j] = fmax(mx[j], x);
+                }
             }
         }
     }
@@ -47,7 +72,7 @@ __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, in
     __shared__ double smn[GB / 64][SJ_MAXK], smx[GB / 64][SJ_MAXK];
     const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < SJ_MAXK; j++) {
+    for (int j = 0; j < KM; j++) {
         double a = mn[j], b = mx[j];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -70,6 +95,21 @@ __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, in
         partial[(uint64_t)blockIdx.x * 2 * k + j] = a;
         partial[(uint64_t)blockIdx.x * 2 * k + k + j] = b;
     }
+}
+
+using MinMaxKernel = void (*)(const double *, uint64_t, int, int, double *, uint32_t *);
+static MinMaxKernel pick_minmax(int d) {
+    switch (d) {
+    case 1: return k_minmax<1>;
+    case 2: return k_minmax<2>;
+    case 3: return k_minmax<3>;
+    case 4: return k_minmax<4>;
+    case 5: return k_minmax<5>;
+    case 6: return k_minmax<6>;
+    case 7: return k_minmax<7>;
+    case 8: return k_minmax<8>;
+    }
+    return k_minmax<0>;
 }
 
 // main.cu:454-501: min/max, then minArr -= eps, maxArr += eps,
@@ -318,17 +358,10 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
     SJ_LAUNCH_CHECK();
     // sorted point IDs (indexLookupArr): the sort's value buffer itself
     g->sortedIds = std::move(vp == vals.p ? vals : vals2);
-    // the error flags and nNonEmpty (d_flags[1], written by the compaction)
-    uint64_t *pin = g->ctx->pinned;
-    SJ_TRY_HIP(hipMemcpyAsync(pin, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipStreamSynchronize(s));
-    const uint32_t fl = (uint32_t)(pin[0] & 0xffffffffu);
-    const uint32_t nne = (uint32_t)(pin[0] >> 32);
-    if (fl & GP_OVERFLOW) {
-        set_error("cell coordinate outside the grid (overflow)");
-        return SJ_ERR_OVERFLOW;
-    }
-    g->nNonEmpty = nne;
+    // the error flags and nNonEmpty (d_flags[1], written by the compaction),
+    // read back by grid_build once the gather behind them is enqueued
+    SJ_TRY_HIP(hipMemcpyAsync(g->ctx->pinned, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipEventRecord(g->ctx->ev, s));
     return SJ_OK;
 }
 
@@ -361,7 +394,7 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     SJ_TRY(flags.alloc(pool, 2));  // error flags, then the non-empty cell count
     SJ_TRY(g->d_gp.alloc(pool, 1));
     SJ_TRY(fill_u32(s, flags.p, 1, 0u));
-    hipLaunchKernelGGL(k_minmax, dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);
+    hipLaunchKernelGGL(pick_minmax(d), dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);
     SJ_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
                        flags.p);
@@ -382,11 +415,22 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     else
         SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags.p));
 
-    const uint64_t nne = g->nNonEmpty;
+    // the coordinates in cell order need no host value: enqueued before the
+    // host waits for the flags and the non-empty cell count
     SJ_TRY(g->sortedPts.alloc(pool, n * (uint64_t)d));
     hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n * (uint64_t)d)), dim3(GB), 0, s, d_points,
                        g->sortedIds.p, n, d, g->sortedPts.p);
     SJ_LAUNCH_CHECK();
+    SJ_TRY_HIP(hipEventSynchronize(ctx->ev));
+    {
+        const uint64_t v = ctx->pinned[0];
+        if ((uint32_t)(v & 0xffffffffu) & GP_OVERFLOW) {
+            set_error("cell coordinate outside the grid (overflow)");
+            return SJ_ERR_OVERFLOW;
+        }
+        g->nNonEmpty = (uint32_t)(v >> 32);
+    }
+    const uint64_t nne = g->nNonEmpty;
     SJ_TRY(g->cellCoord.alloc(pool, nne * (uint64_t)k));
     hipLaunchKernelGGL(k_cell_coords, dim3(blocks_for(nne)), dim3(GB), 0, s, g->sortedPts.p,
                        g->cellStart.p, nne, d, k, eps, g->d_gp.p, g->cellCoord.p);

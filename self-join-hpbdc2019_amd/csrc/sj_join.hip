@@ -510,7 +510,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
 #pragma unroll
                             for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
                         }
-                        L.id[j] = (int32_t)a.origId[t];
+                        if (PASS == PASS_FILL) L.id[j] = (int32_t)a.origId[t];  // emission only
                         L.cnt[j] = 0u;
                     }
                     wave_lds_sync();
@@ -805,6 +805,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
 
 template <int K, int D, int PASS, int PREC, bool RP, class LDS>
 __device__ __forceinline__ void join_body(const JoinArgs &a, LDS *lds) {
+    if (PASS == PASS_FILL && !fill_gate(a)) return;
     const unsigned wave = threadIdx.x >> 6;
     const unsigned lane = __lane_id();
     LDS &L = lds[wave];
@@ -1201,6 +1202,12 @@ __global__ void k_count_scalars(const uint64_t *total, const unsigned long long 
     if (t == 18) out[18] = dirTotal ? *dirTotal : 0ull;
 }
 
+// The fill's error count and result-overflow flag after the count's scalars.
+__global__ void k_fill_scalars(const unsigned long long *stat, uint64_t *out) {
+    if (threadIdx.x == 0) out[19] = stat[STAT_ERR];
+    if (threadIdx.x == 1) out[20] = stat[STAT_OVF];
+}
+
 // The count pass's counters at zero: mirrored-hit counts of the n rows,
 // statistics, and (when not null) the queue slice counters and the
 // hit-record counters.
@@ -1214,11 +1221,14 @@ __global__ __launch_bounds__(JB) void k_zero_count_state(uint32_t *mir, uint64_t
 }
 
 
-// mirrored-hit cursors of rows [w0, w1): start right after the own hits
+// mirrored-hit cursors of rows [w0, w1): start right after the own hits.
+// recBad (when set): only when the hit record is unusable (the fill then
+// evaluates distances again and takes its mirrored slots from the cursors)
 __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
-                              uint64_t outBase, unsigned long long *mirCur) {
+                              uint64_t outBase, unsigned long long *mirCur, const unsigned long long *recBad) {
     uint64_t s = (uint64_t)w0 + (uint64_t)blockIdx.x * JB + threadIdx.x;
     if (s >= w1) return;
+    if (recBad && *recBad == 0ull) return;
     mirCur[s] = offS[s] - outBase + own[s];
 }
 
@@ -1753,7 +1763,8 @@ static sj_status sample_count_list(sj_grid_s *g, const uint32_t *list, uint64_t 
     return SJ_OK;
 }
 
-sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base);
+sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base,
+                           uint64_t idsCap = 0);
 
 // Decision constants of the precision form.  The FP32 LDS tile walk
 // (d == k <= 8) centres each tile in FP64 and sets its band per tile: it
@@ -1937,47 +1948,59 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     }
     SJ_TRY(scan_exclusive_sum2_u32_to_u64(pool, s, own.p, mir.p, n, r->d_offsets.p));
     ts.stop();
-    // |S|, the statistics and the record's state in one readback
+    // |S|, the statistics and the record's state, gathered for one readback.
+    // When the context has seen results of about this size (idsHint) and no
+    // batches are asked for, the fill goes ahead at once into a buffer of
+    // that capacity, gated on the device (fill_gate: the record's state, the
+    // result fitting); the host reads everything once at the end and fills
+    // again only if the result outgrew the buffer.
     uint64_t *pin = ctx->pinned;
     DBuf<uint64_t> scal;
-    SJ_TRY(scal.alloc(pool, 19));
+    SJ_TRY(scal.alloc(pool, 21));
     hipLaunchKernelGGL(k_count_scalars, dim3(1), dim3(64), 0, s, r->d_offsets.p + n, stat.p,
                        recording ? recCtr.p : nullptr, recording ? tileDir.p + tBound : nullptr, scal.p);
     SJ_LAUNCH_CHECK();
-    SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, 19 * 8, hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipStreamSynchronize(s));
+    const bool ahead = P->capacity == 0 && !r->extIds && ctx->idsHint > 0;
     bool replay = false;
-    if (recording) {
-        replay = pin[16 + REC_BAD] == 0;
-        if (!replay) {
-            // grow to the demand seen (at least doubling the mask space, whose
-            // slices may fill unevenly), within a quarter of the free memory
-            size_t freeB = 0, totalB = 0;
-            if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) freeB = 0;
-            const uint64_t wantDir = std::max<uint64_t>(pin[18], a.dirCap);
-            std::vector<unsigned long long> hc(REC_NCTR);
-            SJ_TRY_HIP(hipMemcpy(hc.data(), recCtr.p, REC_NCTR * 8, hipMemcpyDeviceToHost));
-            uint64_t want = 0;
-            for (uint32_t sl = 0; sl < REC_NSLICE; sl++) want += hc[REC_SLICE0 + sl * REC_STRIDE + 1];
-            const uint64_t wantMask = std::max<uint64_t>(want + want / 4 + (uint64_t)REC_NSLICE * REC_BLK * 4,
-                                                         pin[18] > a.dirCap ? a.maskCap : 2 * a.maskCap);
-            if ((wantDir * 8 * REC_DW + wantMask * 8) <= freeB / 4) {
-                ctx->recDirCap = std::max(ctx->recDirCap, wantDir);
-                ctx->recMaskCap = std::max(ctx->recMaskCap, wantMask);
+    // the count pass's scalars (sc: k_count_scalars layout) into the result
+    auto absorb = [&](const uint64_t *sc) -> sj_status {
+        if (recording) {
+            replay = sc[16 + REC_BAD] == 0;
+            if (!replay) {
+                // grow to the demand seen (at least doubling the mask space, whose
+                // slices may fill unevenly), within a quarter of the free memory
+                size_t freeB = 0, totalB = 0;
+                if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) freeB = 0;
+                const uint64_t wantDir = std::max<uint64_t>(sc[18], a.dirCap);
+                std::vector<unsigned long long> hc(REC_NCTR);
+                SJ_TRY_HIP(hipMemcpy(hc.data(), recCtr.p, REC_NCTR * 8, hipMemcpyDeviceToHost));
+                uint64_t want = 0;
+                for (uint32_t sl = 0; sl < REC_NSLICE; sl++) want += hc[REC_SLICE0 + sl * REC_STRIDE + 1];
+                const uint64_t wantMask = std::max<uint64_t>(want + want / 4 + (uint64_t)REC_NSLICE * REC_BLK * 4,
+                                                             sc[18] > a.dirCap ? a.maskCap : 2 * a.maskCap);
+                if ((wantDir * 8 * REC_DW + wantMask * 8) <= freeB / 4) {
+                    ctx->recDirCap = std::max(ctx->recDirCap, wantDir);
+                    ctx->recMaskCap = std::max(ctx->recMaskCap, wantMask);
+                }
             }
         }
+        r->total = sc[0];
+        r->stats.total_pairs = sc[0];
+        r->stats.candidates = sc[1 + STAT_CAND];
+        r->stats.fp32_band_pairs = sc[1 + STAT_BAND];
+        r->stats.fp32_misclassified = sc[1 + STAT_MIS];
+        if (sc[1 + STAT_ERR]) {
+            set_error("join: count pass met %llu inconsistent tiles", (unsigned long long)sc[1 + STAT_ERR]);
+            return SJ_ERR_STATE;
+        }
+        return SJ_OK;
+    };
+    if (!ahead) {
+        SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, 19 * 8, hipMemcpyDeviceToHost, s));
+        SJ_TRY_HIP(hipStreamSynchronize(s));
+        SJ_TRY(absorb(pin));
     }
-    const uint64_t total = pin[0];
-    r->total = total;
-    r->stats.total_pairs = total;
-    r->stats.candidates = pin[1 + STAT_CAND];
-    r->stats.fp32_band_pairs = pin[1 + STAT_BAND];
-    r->stats.fp32_misclassified = pin[1 + STAT_MIS];
     r->stats.overflow_retries = 0;
-    if (pin[1 + STAT_ERR]) {
-        set_error("join: count pass met %llu inconsistent tiles", (unsigned long long)pin[1 + STAT_ERR]);
-        return SJ_ERR_STATE;
-    }
     SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
     // row IDs: a grid made for this call alone hands its sorted IDs over at
     // the end; otherwise they are copied (into the caller's array when it
@@ -1997,47 +2020,86 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     a.mirCur = mirCur.p;
     a.offS = r->d_offsets.p;
 
-    const bool whole = P->capacity == 0 || total <= P->capacity;
+    // the fill of the whole result into ids[cap]: the replayed LDS tile walk
+    // places mirrored hits at the recorded reservations, every other fill
+    // takes them from the cursors.  known: the record's state is known on the
+    // host (replay); otherwise both fills are launched, each gated on it.
+    auto launch_fill = [&](uint64_t cap, bool known) -> sj_status {
+        a.ids = r->d_ids.p;
+        a.outBase = 0;
+        a.idsCap = cap;
+        const bool gate = !known && recording;
+        if (!gate) {
+            if (!(replay && a.mDP == 0)) {
+                hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                                   (uint32_t)n, 0ull, mirCur.p, nullptr);
+                SJ_LAUNCH_CHECK();
+            }
+            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
+            SJ_TRY(launch_join(g, a, PASS_FILL, prec, replay));
+        } else {
+            hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                               (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr);
+            SJ_LAUNCH_CHECK();
+            JoinArgs ar = a;
+            ar.recGate = recCtr.p + REC_BAD;
+            ar.recGateMode = 1;  // replay: the record is usable
+            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
+            SJ_TRY(launch_join(g, ar, PASS_FILL, prec, true));
+            ar.recGateMode = 2;  // evaluate: the record is not
+            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
+            SJ_TRY(launch_join(g, ar, PASS_FILL, prec, false));
+        }
+        return SJ_OK;
+    };
+
+    const bool whole = ahead || P->capacity == 0 || r->total <= P->capacity;
     double ms_fill = 0.0;
     if (whole) {
+        bool known = !ahead;
+        uint64_t cap = r->total;
         if (r->extIds) {
-            if (total > r->extIdsCap) {
+            if (r->total > r->extIdsCap) {
                 set_error("join: %llu result pairs exceed the caller's capacity of %llu",
-                          (unsigned long long)total, (unsigned long long)r->extIdsCap);
+                          (unsigned long long)r->total, (unsigned long long)r->extIdsCap);
                 return SJ_ERR_OVERFLOW;
             }
             r->d_ids.p = r->extIds;
-            r->d_ids.n = total;
+            r->d_ids.n = r->total;
             r->d_ids.pool = nullptr;
         } else {
-            SJ_TRY(r->d_ids.alloc(pool, total));
+            if (ahead) cap = ctx->idsHint;
+            SJ_TRY(r->d_ids.alloc(pool, cap));
         }
-        // the replayed LDS tile walk places mirrored hits at the recorded
-        // reservations; every other fill takes them from the cursors
-        if (!(replay && a.mDP == 0)) {
-            hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                               (uint32_t)n, 0ull, mirCur.p);
+        for (int round = 0; round < 2; round++) {
+            tf.start();
+            SJ_TRY(launch_fill(known ? 0 : cap, known));
+            tf.stop();
+            if (P->sort_rows) {
+                tr.start();
+                SJ_TRY(sort_rows_device(ctx, r->d_offsets.p, r->d_ids.p, n, 0, known ? 0 : cap));
+                tr.stop();
+            }
+            // the fill's errors and overflow after the count's scalars
+            hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, s, stat.p, scal.p);
             SJ_LAUNCH_CHECK();
+            SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, 21 * 8, hipMemcpyDeviceToHost, s));
+            SJ_TRY_HIP(hipStreamSynchronize(s));
+            if (known) break;
+            SJ_TRY(absorb(pin));
+            if (!pin[20]) break;
+            // the result outgrew the buffer: fill again into one of its size
+            known = true;
+            r->stats.overflow_retries = 1;
+            SJ_TRY(r->d_ids.alloc(pool, r->total));
+            SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
         }
-        a.ids = r->d_ids.p;
-        a.outBase = 0;
-        if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
-        tf.start();
-        SJ_TRY(launch_join(g, a, PASS_FILL, prec, replay));
-        tf.stop();
-        r->stats.fill_from_record = replay ? 1 : 0;
-        if (P->sort_rows) {
-            tr.start();
-            SJ_TRY(sort_rows_device(ctx, r->d_offsets.p, r->d_ids.p, n, 0));
-            tr.stop();
-        }
-        SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + STAT_ERR, 8, hipMemcpyDeviceToHost, s));
-        SJ_TRY_HIP(hipStreamSynchronize(s));
-        if (pin[0]) {
+        if (pin[19]) {
             set_error("join: fill pass disagreed with the count pass (%llu writes)",
-                      (unsigned long long)pin[0]);
+                      (unsigned long long)pin[19]);
             return SJ_ERR_STATE;
         }
+        r->stats.fill_from_record = replay ? 1 : 0;
         r->on_device = true;
         r->stats.num_batches = 1;
         ms_fill = tf.ms();
@@ -2132,7 +2194,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             const uint32_t t1 = (uint32_t)(std::lower_bound(hTile.p, hTile.p + nT + 1, (uint32_t)q1) - hTile.p);
             const uint64_t base = hOff.p[s0];
             hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(s1 - s0)), dim3(JB), 0, s, r->d_offsets.p, own.p, s0, s1,
-                               (unsigned long long)base, mirCur.p);
+                               (unsigned long long)base, mirCur.p, nullptr);
             SJ_LAUNCH_CHECK();
             // the buffer's previous download must be done before it is refilled
             if (bi >= 2) SJ_TRY_HIP(hipStreamWaitEvent(s, copied[sl].e, 0));
@@ -2179,6 +2241,8 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         r->stats.fill_from_record = replay ? 1 : 0;
     }
     if (g->transient && !r->extRowIds) r->d_rowIds = std::move(g->sortedIds);
+    // the next join on this context fills ahead into a buffer of about this size
+    if (!r->extIds) ctx->idsHint = std::max<uint64_t>(ctx->idsHint, r->total + r->total / 16 + 4096);
     r->stats.ms_workload = tw.ms();
     r->stats.ms_count = tc.ms();
     r->stats.ms_scan = ts.ms();
@@ -2194,11 +2258,12 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 constexpr int SR_CAP = 1024;
 
 __global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int32_t *ids, uint64_t n,
-                                                   uint64_t base) {
+                                                   uint64_t base, uint64_t idsCap) {
     __shared__ int32_t buf[4][SR_CAP];
     const unsigned wave = threadIdx.x >> 6, lane = __lane_id();
     const uint64_t row = (uint64_t)blockIdx.x * 4 + wave;
     if (row >= n) return;
+    if (idsCap && offsets[n] - base > idsCap) return;  // the fill wrote nothing
     const uint64_t b = offsets[row] - base, e = offsets[row + 1] - base;
     const uint64_t len = e - b;
     if (len < 2) return;
@@ -2236,10 +2301,11 @@ __global__ __launch_bounds__(256) void k_sort_rows(const uint64_t *offsets, int3
     }
 }
 
-sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base) {
+sj_status sort_rows_device(sj_context ctx, const uint64_t *offsets, int32_t *ids, uint64_t n, uint64_t base,
+                           uint64_t idsCap) {
     if (!n) return SJ_OK;
     hipLaunchKernelGGL(k_sort_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, offsets, ids, n,
-                       base);
+                       base, idsCap);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }

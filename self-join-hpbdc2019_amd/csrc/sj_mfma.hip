@@ -9,6 +9,7 @@ namespace sj {
 
 template <int DP, int PASS, bool RP>
 __device__ __forceinline__ void join_body_mfma(const JoinArgs &a, MfmaLds<DP> &L) {
+    if (PASS == PASS_FILL && !fill_gate(a)) return;
     const unsigned lane = __lane_id();
     mfma_lds_clear(L);
     const uint32_t nTiles = *a.nTilesDev;

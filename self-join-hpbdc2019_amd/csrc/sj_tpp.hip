@@ -25,6 +25,7 @@ __global__ __launch_bounds__(JB) void k_join_tpp(JoinArgs a, const uint32_t *ord
     const uint32_t j = lane % T;  // T divides 64
     const uint32_t gbase = lane - j;
     const uint64_t gmask = (T == 64u ? ~0ull : ((1ull << T) - 1ull)) << gbase;
+    if (PASS == PASS_FILL && !fill_gate(a)) return;
     LaneStats st;
     if (gid < count) {
         const int d = a.d;

@@ -14,7 +14,7 @@ constexpr int QSLICES = 8;      // work-queue slices (one counter each)
 constexpr int QGROUP = 8;       // consecutive queue entries a slice takes together
 constexpr int QSTRIDE = 32;     // u32 words between slice counters (128 B)
 enum { PASS_COUNT = 0, PASS_FILL = 1 };
-enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4 };
+enum { STAT_CAND = 0, STAT_BAND = 1, STAT_MIS = 2, STAT_ERR = 3, STAT_FALLBACK = 4, STAT_OVF = 6 };
 
 struct JoinArgs {
     const double *pts;        // n x d, cell order
@@ -43,6 +43,13 @@ struct JoinArgs {
     const uint32_t *tileOrder;  // work queue order of tiles, or null
     uint32_t *qctr;
     const uint32_t *tppOrder;   // threads-per-point schedule: points in workload order, or null
+    // fill launched before the host read the count pass's results: the
+    // result must fit idsCap IDs (0: no check), and recGate (when set)
+    // runs the pass only if the hit record is usable (recGateMode 1) or
+    // unusable (recGateMode 2)
+    uint64_t idsCap;
+    const unsigned long long *recGate;
+    int recGateMode, pad2;
     // rows written by this launch (sorted positions)
     uint32_t wBegin, wEnd;
     // count pass
@@ -206,6 +213,21 @@ __device__ __forceinline__ int64_t find_cell(const JoinArgs &a, uint64_t lin) {
         if (a.cellLinear[mid] < lin) lo = mid + 1; else hi = mid;
     }
     return (lo < a.nne && a.cellLinear[lo] == lin) ? (int64_t)lo : -1;
+}
+
+// Whether a fill launched ahead of the host runs: its result fits the ids
+// buffer (else STAT_OVF is raised and nothing is written) and the hit
+// record is in the state the pass was launched for.  Uniform per launch.
+__device__ __forceinline__ bool fill_gate(const JoinArgs &a) {
+    if (a.recGate) {
+        const bool bad = *a.recGate != 0ull;
+        if (bad == (a.recGateMode == 1)) return false;
+    }
+    if (a.idsCap && a.offS[a.wEnd] - a.outBase > a.idsCap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.stat[STAT_OVF], 1ull);
+        return false;
+    }
+    return true;
 }
 
 // first sorted position whose cell has linear ID >= lin

@@ -7,6 +7,7 @@ namespace sj {
 template <int PASS>
 __global__ __launch_bounds__(JB) void k_join_wide(JoinArgs a) {
     __shared__ WideLds lds[WPB];
+    if (PASS == PASS_FILL && !fill_gate(a)) return;
     const unsigned wave = threadIdx.x >> 6;
     const unsigned lane = __lane_id();
     WideLds &L = lds[wave];
