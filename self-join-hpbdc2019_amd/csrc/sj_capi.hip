@@ -140,7 +140,11 @@ SJ_API sj_status sj_context_create(int32_t device, void *stream, sj_context *out
         return SJ_ERR_NOMEM;
     }
     e = hipEventCreateWithFlags(&c->ev, hipEventDisableTiming);
+    for (int i = 0; i < 10 && e == hipSuccess; i++) e = hipEventCreate(&c->tev[i]);
     if (e != hipSuccess) {
+        for (int i = 0; i < 10; i++)
+            if (c->tev[i]) (void)hipEventDestroy(c->tev[i]);
+        if (c->ev) (void)hipEventDestroy(c->ev);
         (void)hipHostFree(c->pinned);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -173,6 +177,8 @@ SJ_API void sj_context_destroy(sj_context ctx) {
     }
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->ev) (void)hipEventDestroy(ctx->ev);
+    for (int i = 0; i < 10; i++)
+        if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

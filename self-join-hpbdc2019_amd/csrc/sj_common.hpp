@@ -147,6 +147,10 @@ sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *i
                                     uint64_t *out /* n+1 */);
 sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n,
                                     uint32_t *out /* n+1 */);
+// exclusive scan of the run-head flags of sorted keys (1 where keys[i] !=
+// keys[i-1]), total (the number of runs) at out[n]
+sj_status scan_exclusive_heads(Pool *pool, hipStream_t s, const uint32_t *keys, uint64_t n, uint32_t *out);
+sj_status scan_exclusive_heads(Pool *pool, hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t *out);
 // exclusive scan of a[i] + b[i], total at out[n]
 sj_status scan_exclusive_sum2_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *a, const uint32_t *b,
                                          uint64_t n, uint64_t *out /* n+1 */);
@@ -173,6 +177,7 @@ struct sj_context_s {
     hipStream_t copyStream = nullptr;  // result-batch downloads (created on first use)
     uint64_t *pinned = nullptr;  // 64 scalars of pinned readback space
     hipEvent_t ev = nullptr;     // marks a readback into `pinned` the host waits for
+    hipEvent_t tev[10] = {};     // phase timers of the joins (pairs)
     int cu_count = 256;
     // hit-record space of the joins on this context (directory entries,
     // mask words): grows to the demand a join reports when it did not fit

@@ -205,22 +205,15 @@ __global__ __launch_bounds__(GB) void k_cell_keys(const double *pts, uint64_t n,
 }
 
 template <class KT>
-__global__ __launch_bounds__(GB) void k_head_flags(const KT *keys, uint64_t n, uint32_t *head) {
-    uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    if (i >= n) return;
-    head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
-}
-
-template <class KT>
-__global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint32_t *head,
-                                                      const uint32_t *ex, uint64_t n,
+__global__ __launch_bounds__(GB) void k_compact_cells(const KT *keys, const uint32_t *ex, uint64_t n,
                                                       uint32_t *cellOf, uint32_t *cellStart,
                                                       uint64_t *cellLinear, uint32_t *nne) {
     uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
     if (i >= n) return;
-    uint32_t c = ex[i] + head[i] - 1u;
+    const bool head = i == 0 || keys[i] != keys[i - 1];
+    uint32_t c = ex[i] + (head ? 1u : 0u) - 1u;
     cellOf[i] = c;
-    if (head[i]) {
+    if (head) {
         cellStart[c] = (uint32_t)i;
         cellLinear[c] = (uint64_t)keys[i];
     }
@@ -344,17 +337,15 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
         st = radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, n, bits);
     SJ_TRY(st);
 
-    DBuf<uint32_t> head, ex;
-    SJ_TRY(head.alloc(pool, n));
+    // cell index of each sorted position: the scan of the run heads
+    DBuf<uint32_t> ex;
     SJ_TRY(ex.alloc(pool, n + 1));
-    hipLaunchKernelGGL((k_head_flags<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, n, head.p);
-    SJ_LAUNCH_CHECK();
-    SJ_TRY(scan_exclusive_u32_to_u32(pool, s, head.p, n, ex.p));
+    SJ_TRY(scan_exclusive_heads(pool, s, kp, n, ex.p));
     SJ_TRY(g->cellOf.alloc(pool, n));
     SJ_TRY(g->cellStart.alloc(pool, n + 1));
     SJ_TRY(g->cellLinear.alloc(pool, n));
-    hipLaunchKernelGGL((k_compact_cells<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, head.p, ex.p, n,
-                       g->cellOf.p, g->cellStart.p, g->cellLinear.p, d_flags + 1);
+    hipLaunchKernelGGL((k_compact_cells<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, kp, ex.p, n, g->cellOf.p,
+                       g->cellStart.p, g->cellLinear.p, d_flags + 1);
     SJ_LAUNCH_CHECK();
     // sorted point IDs (indexLookupArr): the sort's value buffer itself
     g->sortedIds = std::move(vp == vals.p ? vals : vals2);

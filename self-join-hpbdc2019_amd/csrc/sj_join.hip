@@ -1193,11 +1193,13 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 // The host's view after the count pass, gathered for one copy: out[0] =
 // |S|, out[1..8] = statistics, out[16] = record unusable, out[18] =
 // directory entries the record needed (the last two when recording).
-__global__ void k_count_scalars(const uint64_t *total, const unsigned long long *stat,
+// The count's error counter is cleared once read, for the fill's errors.
+__global__ void k_count_scalars(const uint64_t *total, unsigned long long *stat,
                                 const unsigned long long *recCtr, const uint64_t *dirTotal, uint64_t *out) {
     const unsigned t = threadIdx.x;
     if (t == 0) out[0] = *total;
     if (t >= 1 && t <= 8) out[t] = stat[t - 1];
+    if (t == 1 + STAT_ERR) stat[STAT_ERR] = 0ull;
     if (t == 16) out[16] = recCtr ? recCtr[REC_BAD] : 0ull;
     if (t == 18) out[18] = dirTotal ? *dirTotal : 0ull;
 }
@@ -1224,9 +1226,13 @@ __global__ __launch_bounds__(JB) void k_zero_count_state(uint32_t *mir, uint64_t
 // mirrored-hit cursors of rows [w0, w1): start right after the own hits.
 // recBad (when set): only when the hit record is unusable (the fill then
 // evaluates distances again and takes its mirrored slots from the cursors)
+// qctr (when set): the work-queue counters at zero for the fill that follows.
 __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
-                              uint64_t outBase, unsigned long long *mirCur, const unsigned long long *recBad) {
-    uint64_t s = (uint64_t)w0 + (uint64_t)blockIdx.x * JB + threadIdx.x;
+                              uint64_t outBase, unsigned long long *mirCur, const unsigned long long *recBad,
+                              uint32_t *qctr) {
+    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (qctr && i < (uint64_t)QSLICES * QSTRIDE) qctr[i] = 0u;
+    uint64_t s = (uint64_t)w0 + i;
     if (s >= w1) return;
     if (recBad && *recBad == 0ull) return;
     mirCur[s] = offS[s] - outBase + own[s];
@@ -1416,15 +1422,22 @@ double eps_threshold(double eps) {
 // |sum f^2 - sum a^2| <= E(2 sqrt(D) |a| + D E);
 // FP64 canonical: |d2_64 - |a|^2| <= rho |a|^2, rho = (D+3) u64.
 // d2_64 <= T  =>  s32 <= hi(T/(1-rho));  d2_64 > T  =>  s32 >= lo(T/(1+rho)).
+// Device time between two events on a stream.  The events are the
+// context's (reused by every join) or, when none are given, its own.
 struct EvTimer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
-    bool used = false;
-    explicit EvTimer(hipStream_t st) : s(st) {
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
+    bool used = false, own = false;
+    explicit EvTimer(hipStream_t st, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr) : a(ea), b(eb), s(st) {
+        if (!a || !b) {
+            own = true;
+            a = b = nullptr;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+        }
     }
     ~EvTimer() {
+        if (!own) return;
         if (a) (void)hipEventDestroy(a);
         if (b) (void)hipEventDestroy(b);
     }
@@ -1860,7 +1873,8 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     const double T = eps_threshold(g->eps);
     JoinArgs a = base_args(g, P->form, T);
     if (P->cells_per_tile < 0) a.useTiles = 0;  // per-point walk only (diagnostics)
-    EvTimer tw(s), tc(s), ts(s), tf(s), tr(s);
+    EvTimer tw(s, ctx->tev[0], ctx->tev[1]), tc(s, ctx->tev[2], ctx->tev[3]), ts(s, ctx->tev[4], ctx->tev[5]),
+        tf(s, ctx->tev[6], ctx->tev[7]), tr(s, ctx->tev[8], ctx->tev[9]);
 
     // precision form: FP32 copy relative to the grid origin + margin
     int prec = P->precision;
@@ -2001,7 +2015,6 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         SJ_TRY(absorb(pin));
     }
     r->stats.overflow_retries = 0;
-    SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
     // row IDs: a grid made for this call alone hands its sorted IDs over at
     // the end; otherwise they are copied (into the caller's array when it
     // gave one)
@@ -2029,22 +2042,24 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         a.outBase = 0;
         a.idsCap = cap;
         const bool gate = !known && recording;
+        uint32_t *qz = a.tileOrder ? qctr.p : nullptr;  // queue counters, zeroed by the cursor kernel
+        const unsigned nbq = (unsigned)std::max<uint64_t>(nblk(n), nblk(QSLICES * QSTRIDE));
         if (!gate) {
             if (!(replay && a.mDP == 0)) {
-                hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                                   (uint32_t)n, 0ull, mirCur.p, nullptr);
+                hipLaunchKernelGGL(k_mir_cursors, dim3(nbq), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                                   (uint32_t)n, 0ull, mirCur.p, nullptr, qz);
                 SJ_LAUNCH_CHECK();
+            } else if (qz) {
+                SJ_TRY(fill_u32(s, qz, QSLICES * QSTRIDE, 0u));
             }
-            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
             SJ_TRY(launch_join(g, a, PASS_FILL, prec, replay));
         } else {
-            hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(n)), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                               (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr);
+            hipLaunchKernelGGL(k_mir_cursors, dim3(nbq), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
+                               (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr, qz);
             SJ_LAUNCH_CHECK();
             JoinArgs ar = a;
             ar.recGate = recCtr.p + REC_BAD;
             ar.recGateMode = 1;  // replay: the record is usable
-            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
             SJ_TRY(launch_join(g, ar, PASS_FILL, prec, true));
             ar.recGateMode = 2;  // evaluate: the record is not
             if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
@@ -2194,7 +2209,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             const uint32_t t1 = (uint32_t)(std::lower_bound(hTile.p, hTile.p + nT + 1, (uint32_t)q1) - hTile.p);
             const uint64_t base = hOff.p[s0];
             hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(s1 - s0)), dim3(JB), 0, s, r->d_offsets.p, own.p, s0, s1,
-                               (unsigned long long)base, mirCur.p, nullptr);
+                               (unsigned long long)base, mirCur.p, nullptr, nullptr);
             SJ_LAUNCH_CHECK();
             // the buffer's previous download must be done before it is refilled
             if (bi >= 2) SJ_TRY_HIP(hipStreamWaitEvent(s, copied[sl].e, 0));

@@ -28,6 +28,13 @@ struct In2 {
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return (uint64_t)a[i] + b[i]; }
 };
 
+// head flag of a sorted key array: 1 where a run of equal keys starts
+template <class K>
+struct InHead {
+    const K *k;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return i == 0 || k[i] != k[i - 1] ? 1u : 0u; }
+};
+
 template <class T>
 __device__ __forceinline__ T wave_incl(T v) {
     const unsigned lane = __lane_id();
@@ -189,6 +196,14 @@ sj_status scan_exclusive_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *i
 
 sj_status scan_exclusive_u32_to_u32(Pool *pool, hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out) {
     return scan_two_pass<In1, uint32_t>(pool, s, In1{in}, n, out);
+}
+
+sj_status scan_exclusive_heads(Pool *pool, hipStream_t s, const uint32_t *keys, uint64_t n, uint32_t *out) {
+    return scan_two_pass<InHead<uint32_t>, uint32_t>(pool, s, InHead<uint32_t>{keys}, n, out);
+}
+
+sj_status scan_exclusive_heads(Pool *pool, hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t *out) {
+    return scan_two_pass<InHead<uint64_t>, uint32_t>(pool, s, InHead<uint64_t>{keys}, n, out);
 }
 
 }  // namespace sj
