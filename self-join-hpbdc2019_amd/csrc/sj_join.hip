@@ -383,412 +383,478 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     // matter only to tiles that own rows there
     const bool anyOwnW = __ballot(ownW) != 0ull;
 
-    // neighbour rows: odometer over dims 1..K-1
-    int o[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) o[j] = -1;
-    o[0] = 0;
+    // union of the lanes' three-cell windows in dim 0
+    const uint32_t ux0 = xa >= 1 ? xa - 1 : 0u;
+    const uint32_t ux1 = xb + 1 < nC0 ? xb + 1 : nC0 - 1;
     int nrow = 1;
 #pragma unroll
     for (int j = 1; j < K; j++) nrow *= 3;
-    for (int it = 0; it < nrow; ++it) {
-        bool inb = true, originRow = true;
-        int64_t dl = 0;
+
+    // One neighbour row at offsets o[1..K-1] (o[0] unused) with cell-row
+    // base rb; the caller may pass the union's bounds [pu0, pu1) (pre).
+    // Every lane shares the row coordinates, so control flow is uniform.
+    auto row = [&](const int (&o)[K], uint64_t rb, bool originRow, bool pre, uint32_t pu0,
+                   uint32_t pu1) __attribute__((always_inline)) {
+        // claimed dim-0 offsets of this lane's cell in this row
+        bool useL = false, useC = false, useR = false;
+        if (valid) {
+            int oo[K];
 #pragma unroll
-        for (int j = 1; j < K; j++) {
-            originRow = originRow && o[j] == 0;
-            const int64_t v = (int64_t)c[j] + o[j];
-            inb = inb && v >= 0 && v < (int64_t)a.nCells[j];
-            dl += (int64_t)o[j] * a.mult[j];
+            for (int j = 0; j < K; j++) oo[j] = o[j];
+            oo[0] = -1;
+            useL = x >= 1 && claims<K>(a.form, c, oo, linA, rb + x - 1);
+            oo[0] = 1;
+            useR = x + 1 < nC0 && claims<K>(a.form, c, oo, linA, rb + x + 1);
+            oo[0] = 0;
+            useC = originRow || claims<K>(a.form, c, oo, linA, rb + x);
         }
-        // inb is uniform: every lane shares the row coordinates
-        if (inb) {
-            const uint64_t rb = (uint64_t)((int64_t)rowBase + dl);
-            // claimed dim-0 offsets of this lane's cell in this row
-            bool useL = false, useC = false, useR = false;
+        if (__ballot(useL || useC || useR)) {
+            const uint32_t u0 = pre ? pu0 : pos_lb(a, rb + ux0);
+            const uint32_t u1 = pre ? pu1 : pos_lb(a, rb + ux1 + 1);
+            uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
             if (valid) {
-                int oo[K];
-#pragma unroll
-                for (int j = 0; j < K; j++) oo[j] = o[j];
-                oo[0] = -1;
-                useL = x >= 1 && claims<K>(a.form, c, oo, linA, rb + x - 1);
-                oo[0] = 1;
-                useR = x + 1 < nC0 && claims<K>(a.form, c, oo, linA, rb + x + 1);
-                oo[0] = 0;
-                useC = originRow || claims<K>(a.form, c, oo, linA, rb + x);
+                e1 = pos_lb(a, rb + x);
+                e2 = pos_lb(a, rb + x + 1);
+                e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
+                e3 = x + 1 < nC0 ? pos_lb(a, rb + x + 2) : e2;
             }
-            if (__ballot(useL || useC || useR)) {
-                const uint32_t ux0 = xa >= 1 ? xa - 1 : 0u;
-                const uint32_t ux1 = xb + 1 < nC0 ? xb + 1 : nC0 - 1;
-                const uint32_t u0 = pos_lb(a, rb + ux0);
-                const uint32_t u1 = pos_lb(a, rb + ux1 + 1);
-                uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
-                if (valid) {
-                    e1 = pos_lb(a, rb + x);
-                    e2 = pos_lb(a, rb + x + 1);
-                    e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
-                    e3 = x + 1 < nC0 ? pos_lb(a, rb + x + 2) : e2;
+            // this lane's span: from the first to the last claimed cell;
+            // unclaimed cells inside it are masked per candidate
+            uint32_t lo = useL ? e0 : (useC ? e1 : e2);
+            uint32_t hi = useR ? e3 : (useC ? e2 : e1);
+            if (!(useL || useC || useR)) lo = hi = e0;
+            if (PASS == PASS_COUNT && valid)
+                cand += (useL ? e1 - e0 : 0) + (useC ? e2 - e1 : 0) + (useR ? e3 - e2 : 0);
+            const bool mirC = anyMirror && !originRow;
+            for (uint32_t c0 = u0; c0 < u1; c0 += CH) {
+                const uint32_t m = min((uint32_t)CH, u1 - c0);
+                const uint32_t clo = max(lo, c0);
+                const uint32_t chi = min(hi, c0 + m);
+                const uint32_t len = chi > clo ? chi - clo : 0u;
+                uint32_t maxLen = len;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) maxLen = max(maxLen, (uint32_t)__shfl_xor(maxLen, off, 64));
+                // replay: the chunk's recorded masks, requested before the
+                // staging so that both loads are in flight together
+                uint64_t rm[Cfg::MW];
+#pragma unroll
+                for (int w = 0; w < Cfg::MW; w++) rm[w] = 0ull;
+                const uint32_t np = (maxLen + 63u) / 64u;  // pieces of the chunk
+                const uint64_t chunkIt = piece_it;
+                if (PASS == PASS_FILL && !anyOwnW && (c0 + m <= a.wBegin || c0 >= a.wEnd)) {
+                    piece_it += np;  // nothing of this chunk lands in the rows written
+                    continue;
                 }
-                // this lane's span: from the first to the last claimed cell;
-                // unclaimed cells inside it are masked per candidate
-                uint32_t lo = useL ? e0 : (useC ? e1 : e2);
-                uint32_t hi = useR ? e3 : (useC ? e2 : e1);
-                if (!(useL || useC || useR)) lo = hi = e0;
-                if (PASS == PASS_COUNT && valid)
-                    cand += (useL ? e1 - e0 : 0) + (useC ? e2 - e1 : 0) + (useR ? e3 - e2 : 0);
-                const bool mirC = anyMirror && !originRow;
-                for (uint32_t c0 = u0; c0 < u1; c0 += CH) {
-                    const uint32_t m = min((uint32_t)CH, u1 - c0);
-                    const uint32_t clo = max(lo, c0);
-                    const uint32_t chi = min(hi, c0 + m);
-                    const uint32_t len = chi > clo ? chi - clo : 0u;
-                    uint32_t maxLen = len;
+                if constexpr (RP) {
+                    uint64_t anyHit = 0ull;
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) maxLen = max(maxLen, (uint32_t)__shfl_xor(maxLen, off, 64));
-                    // replay: the chunk's recorded masks, requested before the
-                    // staging so that both loads are in flight together
-                    uint64_t rm[Cfg::MW];
-#pragma unroll
-                    for (int w = 0; w < Cfg::MW; w++) rm[w] = 0ull;
-                    const uint32_t np = (maxLen + 63u) / 64u;  // pieces of the chunk
-                    const uint64_t chunkIt = piece_it;
-                    if (PASS == PASS_FILL && !anyOwnW && (c0 + m <= a.wBegin || c0 >= a.wEnd)) {
-                        piece_it += np;  // nothing of this chunk lands in the rows written
+                    for (int w = 0; w < Cfg::MW; w++) {
+                        if ((uint32_t)w < np) {
+                            const uint64_t *e = a.recDir + REC_DW * (dbase + piece_it + (uint64_t)w);
+                            const uint64_t pres = e[0];
+                            anyHit |= pres;
+                            if ((pres >> lane) & 1ull) rm[w] = a.recMask[e[1] + popc_below64(pres)];
+                        }
+                    }
+                    // no hits in the chunk: nothing to stage or emit
+                    if (!anyHit) {
+                        piece_it += np;
                         continue;
                     }
-                    if constexpr (RP) {
-                        uint64_t anyHit = 0ull;
+                    // mirrored hits: each candidate's range comes from the
+                    // reservation the count pass recorded, packed as
+                    // (row position << 7) | hits
+                    const uint64_t mo = anyMirror ? a.recDir[REC_DW * (dbase + piece_it) + 2] : 0ull;
+                    if (mo) {
+                        const uint64_t *mr = a.recMask + (mo - 1);
+                        uint32_t pre = Cfg::MW;
 #pragma unroll
                         for (int w = 0; w < Cfg::MW; w++) {
-                            if ((uint32_t)w < np) {
-                                const uint64_t *e = a.recDir + REC_DW * (dbase + piece_it + (uint64_t)w);
-                                const uint64_t pres = e[0];
-                                anyHit |= pres;
-                                if ((pres >> lane) & 1ull) rm[w] = a.recMask[e[1] + popc_below64(pres)];
-                            }
-                        }
-                        // no hits in the chunk: nothing to stage or emit
-                        if (!anyHit) {
-                            piece_it += np;
-                            continue;
-                        }
-                        // mirrored hits: each candidate's range comes from the
-                        // reservation the count pass recorded, packed as
-                        // (row position << 7) | hits
-                        const uint64_t mo = anyMirror ? a.recDir[REC_DW * (dbase + piece_it) + 2] : 0ull;
-                        if (mo) {
-                            const uint64_t *mr = a.recMask + (mo - 1);
-                            uint32_t pre = Cfg::MW;
-#pragma unroll
-                            for (int w = 0; w < Cfg::MW; w++) {
-                                const uint64_t hv = mr[w];
-                                if ((hv >> lane) & 1ull) {
-                                    const uint64_t rv = mr[pre + popc_below64(hv)];
-                                    const uint32_t j = lane + 64u * (uint32_t)w;
-                                    const uint64_t row = (uint64_t)c0 + j;
-                                    unsigned long long bs = NO_SLOT;
-                                    if (row >= a.wBegin && row < a.wEnd) {
-                                        const uint64_t v = rv & 255ull;
-                                        const uint64_t b0 = a.offS[row] - a.outBase + a.ownCnt[row] + (rv >> 8);
-                                        if (b0 + v <= a.offS[row + 1] - a.outBase) bs = (b0 << 7) | v;
-                                        else err++;
-                                    }
-                                    L.base[j] = bs;
+                            const uint64_t hv = mr[w];
+                            if ((hv >> lane) & 1ull) {
+                                const uint64_t rv = mr[pre + popc_below64(hv)];
+                                const uint32_t j = lane + 64u * (uint32_t)w;
+                                const uint64_t row = (uint64_t)c0 + j;
+                                unsigned long long bs = NO_SLOT;
+                                if (row >= a.wBegin && row < a.wEnd) {
+                                    const uint64_t v = rv & 255ull;
+                                    const uint64_t b0 = a.offS[row] - a.outBase + a.ownCnt[row] + (rv >> 8);
+                                    if (b0 + v <= a.offS[row + 1] - a.outBase) bs = (b0 << 7) | v;
+                                    else err++;
                                 }
-                                pre += (uint32_t)__popcll(hv);
+                                L.base[j] = bs;
+                            }
+                            pre += (uint32_t)__popcll(hv);
+                        }
+                    }
+                }
+                // stage the chunk
+                for (uint32_t j = lane; j < m; j += 64) {
+                    const uint64_t t = (uint64_t)c0 + j;
+                    if constexpr (RP) {
+                    } else if constexpr (PREC == 1) {
+#pragma unroll
+                        for (int q = 0; q < DD; q++) L.x32[j][q] = (float)(a.pts[t * DD + q] - ctr[q]);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
+                    }
+                    if (PASS == PASS_FILL) L.id[j] = (int32_t)a.origId[t];  // emission only
+                    L.cnt[j] = 0u;
+                }
+                wave_lds_sync();
+                // mirrored hits of the chunk, one word per 64-candidate piece
+                uint64_t mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
+                bool anyMir = false;
+                uint32_t piece = 0;
+                for (uint32_t base = 0; base < maxLen; base += 64, piece++, piece_it++) {
+                    const uint32_t n = min(64u, maxLen - base);
+                    const uint32_t pstart = clo + base;
+                    uint64_t mask = 0ull, maybe = 0ull;
+                    if constexpr (RP) {
+#pragma unroll
+                        for (int w = 0; w < Cfg::MW; w++)
+                            if ((uint32_t)w == piece) mask = rm[w];
+                    } else {
+                    // claimed candidates inside this lane's span, as bits
+                    // of the piece; lanes without work read row 0
+                    uint64_t okm = 0ull;
+                    uint32_t r0 = 0u;
+                    if (base < len) {
+                        if (useL) okm |= range_bits(pstart, e0, e1);
+                        if (useC) okm |= range_bits(pstart, e1, e2);
+                        if (useR) okm |= range_bits(pstart, e2, e3);
+                        okm &= range_bits(pstart, pstart, clo + len);
+                        r0 = pstart - c0;
+                    }
+                    // uniform trip count, US candidates per step, one bit
+                    // per candidate; masks are applied per piece
+                    constexpr int US = D <= 3 ? 8 : 4;
+                    for (uint32_t i = 0; i < n; i += US) {
+                        uint32_t hb = 0u, mb = 0u;
+#pragma unroll
+                        for (int u = 0; u < US; u++) {
+                            const uint32_t row = r0 + i + (uint32_t)u;
+                            if constexpr (PREC == 1) {
+                                float q[DD];
+#pragma unroll
+                                for (int z = 0; z < DD; z++) q[z] = L.x32[row][z];
+                                const float s32 = d2_fp32<D>(p32, q, DD);
+                                hb |= (s32 < lo32 ? 1u : 0u) << u;
+                                mb |= (s32 <= hi32 ? 1u : 0u) << u;
+                            } else {
+                                double q[DD];
+#pragma unroll
+                                for (int z = 0; z < DD; z++) q[z] = L.x64[row][z];
+                                hb |= (d2_fp64<D>(p, q, DD) <= a.T ? 1u : 0u) << u;
                             }
                         }
+                        mask |= (uint64_t)hb << i;
+                        if constexpr (PREC == 1) maybe |= (uint64_t)mb << i;
                     }
-                    // stage the chunk
-                    for (uint32_t j = lane; j < m; j += 64) {
-                        const uint64_t t = (uint64_t)c0 + j;
-                        if constexpr (RP) {
-                        } else if constexpr (PREC == 1) {
-#pragma unroll
-                            for (int q = 0; q < DD; q++) L.x32[j][q] = (float)(a.pts[t * DD + q] - ctr[q]);
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
-                        }
-                        if (PASS == PASS_FILL) L.id[j] = (int32_t)a.origId[t];  // emission only
-                        L.cnt[j] = 0u;
-                    }
-                    wave_lds_sync();
-                    // mirrored hits of the chunk, one word per 64-candidate piece
-                    uint64_t mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
-                    bool anyMir = false;
-                    uint32_t piece = 0;
-                    for (uint32_t base = 0; base < maxLen; base += 64, piece++, piece_it++) {
-                        const uint32_t n = min(64u, maxLen - base);
-                        const uint32_t pstart = clo + base;
-                        uint64_t mask = 0ull, maybe = 0ull;
-                        if constexpr (RP) {
-#pragma unroll
-                            for (int w = 0; w < Cfg::MW; w++)
-                                if ((uint32_t)w == piece) mask = rm[w];
-                        } else {
-                        // claimed candidates inside this lane's span, as bits
-                        // of the piece; lanes without work read row 0
-                        uint64_t okm = 0ull;
-                        uint32_t r0 = 0u;
-                        if (base < len) {
-                            if (useL) okm |= range_bits(pstart, e0, e1);
-                            if (useC) okm |= range_bits(pstart, e1, e2);
-                            if (useR) okm |= range_bits(pstart, e2, e3);
-                            okm &= range_bits(pstart, pstart, clo + len);
-                            r0 = pstart - c0;
-                        }
-                        // uniform trip count, US candidates per step, one bit
-                        // per candidate; masks are applied per piece
-                        constexpr int US = D <= 3 ? 8 : 4;
-                        for (uint32_t i = 0; i < n; i += US) {
-                            uint32_t hb = 0u, mb = 0u;
-#pragma unroll
-                            for (int u = 0; u < US; u++) {
-                                const uint32_t row = r0 + i + (uint32_t)u;
-                                if constexpr (PREC == 1) {
+                    mask &= okm;
+                    if constexpr (PREC == 1) {
+                        // candidates in [T - M, T + M]: canonical FP64 predicate
+                        const uint64_t bnd = maybe & ~mask & okm;
+                        if (__ballot(bnd != 0ull)) {
+                            uint64_t exm = 0ull, bb = bnd;
+                            while (bb) {
+                                const int bi = __builtin_ctzll(bb);
+                                bb &= bb - 1ull;
+                                const uint64_t t = (uint64_t)pstart + (uint32_t)bi;
+                                const bool ex = d2_fp64<D>(a.pts + (uint64_t)s * DD, a.pts + t * DD, DD) <= a.T;
+                                if (ex) exm |= 1ull << bi;
+                                if (PASS == PASS_COUNT) {
+                                    // the plain FP32 decision, for the count of
+                                    // pairs it would have misclassified
                                     float q[DD];
 #pragma unroll
-                                    for (int z = 0; z < DD; z++) q[z] = L.x32[row][z];
-                                    const float s32 = d2_fp32<D>(p32, q, DD);
-                                    hb |= (s32 < lo32 ? 1u : 0u) << u;
-                                    mb |= (s32 <= hi32 ? 1u : 0u) << u;
-                                } else {
-                                    double q[DD];
-#pragma unroll
-                                    for (int z = 0; z < DD; z++) q[z] = L.x64[row][z];
-                                    hb |= (d2_fp64<D>(p, q, DD) <= a.T ? 1u : 0u) << u;
+                                    for (int z = 0; z < DD; z++) q[z] = L.x32[r0 + (uint32_t)bi][z];
+                                    mis += (d2_fp32<D>(p32, q, DD) <= t32) != ex ? 1u : 0u;
                                 }
                             }
-                            mask |= (uint64_t)hb << i;
-                            if constexpr (PREC == 1) maybe |= (uint64_t)mb << i;
-                        }
-                        mask &= okm;
-                        if constexpr (PREC == 1) {
-                            // candidates in [T - M, T + M]: canonical FP64 predicate
-                            const uint64_t bnd = maybe & ~mask & okm;
-                            if (__ballot(bnd != 0ull)) {
-                                uint64_t exm = 0ull, bb = bnd;
-                                while (bb) {
-                                    const int bi = __builtin_ctzll(bb);
-                                    bb &= bb - 1ull;
-                                    const uint64_t t = (uint64_t)pstart + (uint32_t)bi;
-                                    const bool ex = d2_fp64<D>(a.pts + (uint64_t)s * DD, a.pts + t * DD, DD) <= a.T;
-                                    if (ex) exm |= 1ull << bi;
-                                    if (PASS == PASS_COUNT) {
-                                        // the plain FP32 decision, for the count of
-                                        // pairs it would have misclassified
-                                        float q[DD];
-#pragma unroll
-                                        for (int z = 0; z < DD; z++) q[z] = L.x32[r0 + (uint32_t)bi][z];
-                                        mis += (d2_fp32<D>(p32, q, DD) <= t32) != ex ? 1u : 0u;
-                                    }
-                                }
-                                mask |= exm;
-                                if (PASS == PASS_COUNT) band += (uint64_t)__popcll(bnd);
-                            }
-                        }
-                        }  // distance evaluation
-                        if (PASS == PASS_COUNT && recOn) record_piece(a, R, rec, dbase, dend, piece_it, mask);
-                        // emission: mirrored hits (claimed non-own cells inside
-                        // the rows written) and own hits of the piece
-                        uint64_t mirBits = 0ull;
-                        if (anyMirror) {
-                            uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
-                            if (mirC) elig |= range_bits(pstart, e1, e2);
-                            mirBits = mask & elig & range_bits(pstart, a.wBegin, a.wEnd);
-                        }
-                        const uint32_t myCnt = (uint32_t)__popcll(mask);
-                        uint64_t ownBits = 0ull;
-                        if (PASS == PASS_FILL && ownW) {
-                            // more hits than the row has room for: count and
-                            // fill disagree; drop the piece and report it
-                            if (ownPos + own + myCnt <= ownLim) ownBits = mask;
-                            else err++;
-                        }
-                        const uint64_t wcur = ownPos + own;
-                        const uint32_t jb = pstart - c0;  // LDS row of the piece start (lanes with bits)
-                        // mirrored hits, lane by lane: the count and the
-                        // evaluating fill tally them per candidate; the
-                        // replayed fill places them in the recorded ranges
-                        if (anyMirror) {
-                            uint64_t mb = mirBits;
-                            while (mb) {
-                                const int bi = __builtin_ctzll(mb);
-                                mb &= mb - 1ull;
-                                const uint32_t jj = min(jb + (uint32_t)bi, (uint32_t)(CH - 1));
-                                if constexpr (RP) {
-                                    const uint32_t k2 = atomicAdd(&L.cnt[jj], 1u);
-                                    const unsigned long long bs = L.base[jj];
-                                    if (bs != NO_SLOT && k2 < (uint32_t)(bs & 127ull)) a.ids[(bs >> 7) + k2] = myId;
-                                    else if (bs != NO_SLOT) err++;
-                                } else {
-                                    atomicAdd(&L.cnt[jj], 1u);
-                                }
-                            }
-                        }
-                        // own hits: lane by lane when no lane has more than 24
-                        // (measured best from 8 to 24 at 2-D to 6-D), else
-                        // one source lane at a time, its mask broadcast so that
-                        // each thread takes one candidate bit and the hits
-                        // leave as one contiguous row write
-                        uint32_t mp = PASS == PASS_FILL ? (uint32_t)__popcll(ownBits) : 0u;
-                        if (PASS == PASS_FILL) {
-#pragma unroll
-                            for (int off = 32; off > 0; off >>= 1) mp = max(mp, (uint32_t)__shfl_xor(mp, off, 64));
-                        }
-                        if (PASS == PASS_FILL && mp <= 24u) {
-                            uint64_t ob = ownBits;
-                            uint64_t wp = wcur;
-                            while (ob) {
-                                const int bi = __builtin_ctzll(ob);
-                                ob &= ob - 1ull;
-                                a.ids[wp++] = L.id[min(jb + (uint32_t)bi, (uint32_t)(CH - 1))];
-                            }
-                        } else if (PASS == PASS_FILL) {
-                            uint64_t todo = __ballot(ownBits != 0ull);
-                            // four source lanes per round so that their LDS reads overlap
-                            while (todo) {
-                                int l[4];
-                                bool lv[4];
-#pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    lv[u] = todo != 0ull;
-                                    l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
-                                    todo &= todo - 1ull;
-                                }
-                                uint64_t oB[4];
-                                uint32_t j[4];
-                                int32_t idv[4];
-                                uint64_t w[4];
-#pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    oB[u] = lv[u] ? readlane_u64(ownBits, l[u]) : 0ull;
-                                    j[u] = min(readlane_u32(pstart, l[u]) + lane - c0, (uint32_t)(CH - 1));
-                                }
-#pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    idv[u] = L.id[j[u]];
-                                    w[u] = readlane_u64(wcur, l[u]) + popc_below64(oB[u]);
-                                }
-#pragma unroll
-                                for (int u = 0; u < 4; u++)
-                                    if ((oB[u] >> lane) & 1ull) a.ids[w[u]] = idv[u];
-                            }
-                        }
-                        if (ownW) own += myCnt;
-                        if (PASS == PASS_FILL && mirBits) {
-                            anyMir = true;
-                            if (piece == 0) mk0 = mirBits;
-                            else if (piece == 1) mk1 = mirBits;
-                            else if (piece == 2) mk2 = mirBits;
-                            else mk3 = mirBits;
+                            mask |= exm;
+                            if (PASS == PASS_COUNT) band += (uint64_t)__popcll(bnd);
                         }
                     }
-                    wave_lds_sync();
+                    }  // distance evaluation
+                    if (PASS == PASS_COUNT && recOn) record_piece(a, R, rec, dbase, dend, piece_it, mask);
+                    // emission: mirrored hits (claimed non-own cells inside
+                    // the rows written) and own hits of the piece
+                    uint64_t mirBits = 0ull;
                     if (anyMirror) {
-                        if (PASS == PASS_COUNT) {
-                            // reserve the chunk's mirrored hits of each candidate;
-                            // a record keeps each reservation for the fill
-                            uint64_t hv[Cfg::MW], rv[Cfg::MW];
-                            uint32_t nzm = Cfg::MW;
+                        uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
+                        if (mirC) elig |= range_bits(pstart, e1, e2);
+                        mirBits = mask & elig & range_bits(pstart, a.wBegin, a.wEnd);
+                    }
+                    const uint32_t myCnt = (uint32_t)__popcll(mask);
+                    uint64_t ownBits = 0ull;
+                    if (PASS == PASS_FILL && ownW) {
+                        // more hits than the row has room for: count and
+                        // fill disagree; drop the piece and report it
+                        if (ownPos + own + myCnt <= ownLim) ownBits = mask;
+                        else err++;
+                    }
+                    const uint64_t wcur = ownPos + own;
+                    const uint32_t jb = pstart - c0;  // LDS row of the piece start (lanes with bits)
+                    // mirrored hits, lane by lane: the count and the
+                    // evaluating fill tally them per candidate; the
+                    // replayed fill places them in the recorded ranges
+                    if (anyMirror) {
+                        uint64_t mb = mirBits;
+                        while (mb) {
+                            const int bi = __builtin_ctzll(mb);
+                            mb &= mb - 1ull;
+                            const uint32_t jj = min(jb + (uint32_t)bi, (uint32_t)(CH - 1));
+                            if constexpr (RP) {
+                                const uint32_t k2 = atomicAdd(&L.cnt[jj], 1u);
+                                const unsigned long long bs = L.base[jj];
+                                if (bs != NO_SLOT && k2 < (uint32_t)(bs & 127ull)) a.ids[(bs >> 7) + k2] = myId;
+                                else if (bs != NO_SLOT) err++;
+                            } else {
+                                atomicAdd(&L.cnt[jj], 1u);
+                            }
+                        }
+                    }
+                    // own hits: lane by lane when no lane has more than 24
+                    // (measured best from 8 to 24 at 2-D to 6-D), else
+                    // one source lane at a time, its mask broadcast so that
+                    // each thread takes one candidate bit and the hits
+                    // leave as one contiguous row write
+                    uint32_t mp = PASS == PASS_FILL ? (uint32_t)__popcll(ownBits) : 0u;
+                    if (PASS == PASS_FILL) {
 #pragma unroll
-                            for (int w = 0; w < Cfg::MW; w++) {
-                                const uint32_t j = lane + 64u * (uint32_t)w;
-                                const uint32_t v = j < m ? L.cnt[j] : 0u;
-                                uint32_t b = 0;
-                                if (v) b = atomicAdd(&a.mirCnt[c0 + j], v);
-                                hv[w] = __ballot(v != 0u);
-                                rv[w] = ((uint64_t)b << 8) | v;
-                                nzm += (uint32_t)__popcll(hv[w]);
-                            }
-                            if (recOn && rec && np > 0 && nzm > (uint32_t)Cfg::MW) {
-                                if (rec_room(a, R, nzm)) {
-                                    const uint64_t off = R.blk + R.used;
-                                    uint32_t pre = Cfg::MW;
+                        for (int off = 32; off > 0; off >>= 1) mp = max(mp, (uint32_t)__shfl_xor(mp, off, 64));
+                    }
+                    if (PASS == PASS_FILL && mp <= 24u) {
+                        uint64_t ob = ownBits;
+                        uint64_t wp = wcur;
+                        while (ob) {
+                            const int bi = __builtin_ctzll(ob);
+                            ob &= ob - 1ull;
+                            a.ids[wp++] = L.id[min(jb + (uint32_t)bi, (uint32_t)(CH - 1))];
+                        }
+                    } else if (PASS == PASS_FILL) {
+                        uint64_t todo = __ballot(ownBits != 0ull);
+                        // four source lanes per round so that their LDS reads overlap
+                        while (todo) {
+                            int l[4];
+                            bool lv[4];
 #pragma unroll
-                                    for (int w = 0; w < Cfg::MW; w++) {
-                                        if (lane == (unsigned)w) a.recMask[off + w] = hv[w];
-                                        if ((hv[w] >> lane) & 1ull) a.recMask[off + pre + popc_below64(hv[w])] = rv[w];
-                                        pre += (uint32_t)__popcll(hv[w]);
-                                    }
-                                    R.used += nzm;
-                                    if (lane == 0) a.recDir[REC_DW * (dbase + chunkIt) + 2] = off + 1;
-                                }
+                            for (int u = 0; u < 4; u++) {
+                                lv[u] = todo != 0ull;
+                                l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+                                todo &= todo - 1ull;
                             }
-                            R.want += nzm > (uint32_t)Cfg::MW ? nzm : 0u;
-                        } else if constexpr (!RP) {
-                            // reserve one range per candidate, then place; a
-                            // reservation past the candidate's row end is an
-                            // error and its hits are dropped
-                            for (uint32_t j = lane; j < m; j += 64) {
-                                const uint32_t v = L.cnt[j];
-                                if (v) {
-                                    unsigned long long b = atomicAdd(&a.mirCur[c0 + j], (unsigned long long)v);
-                                    if (b + v > a.offS[c0 + j + 1] - a.outBase) {
-                                        err++;
-                                        b = NO_SLOT;
-                                    }
-                                    L.base[j] = b;
-                                }
-                                L.cnt[j] = 0u;
+                            uint64_t oB[4];
+                            uint32_t j[4];
+                            int32_t idv[4];
+                            uint64_t w[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                oB[u] = lv[u] ? readlane_u64(ownBits, l[u]) : 0ull;
+                                j[u] = min(readlane_u32(pstart, l[u]) + lane - c0, (uint32_t)(CH - 1));
                             }
-                            wave_lds_sync();
-                            // place the mirrored hits, one source lane at a time
-                            if (__ballot(anyMir)) {
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                idv[u] = L.id[j[u]];
+                                w[u] = readlane_u64(wcur, l[u]) + popc_below64(oB[u]);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; u++)
+                                if ((oB[u] >> lane) & 1ull) a.ids[w[u]] = idv[u];
+                        }
+                    }
+                    if (ownW) own += myCnt;
+                    if (PASS == PASS_FILL && mirBits) {
+                        anyMir = true;
+                        if (piece == 0) mk0 = mirBits;
+                        else if (piece == 1) mk1 = mirBits;
+                        else if (piece == 2) mk2 = mirBits;
+                        else mk3 = mirBits;
+                    }
+                }
+                wave_lds_sync();
+                if (anyMirror) {
+                    if (PASS == PASS_COUNT) {
+                        // reserve the chunk's mirrored hits of each candidate;
+                        // a record keeps each reservation for the fill
+                        uint64_t hv[Cfg::MW], rv[Cfg::MW];
+                        uint32_t nzm = Cfg::MW;
+#pragma unroll
+                        for (int w = 0; w < Cfg::MW; w++) {
+                            const uint32_t j = lane + 64u * (uint32_t)w;
+                            const uint32_t v = j < m ? L.cnt[j] : 0u;
+                            uint32_t b = 0;
+                            if (v) b = atomicAdd(&a.mirCnt[c0 + j], v);
+                            hv[w] = __ballot(v != 0u);
+                            rv[w] = ((uint64_t)b << 8) | v;
+                            nzm += (uint32_t)__popcll(hv[w]);
+                        }
+                        if (recOn && rec && np > 0 && nzm > (uint32_t)Cfg::MW) {
+                            if (rec_room(a, R, nzm)) {
+                                const uint64_t off = R.blk + R.used;
+                                uint32_t pre = Cfg::MW;
 #pragma unroll
                                 for (int w = 0; w < Cfg::MW; w++) {
-                                    const uint64_t mb = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
-                                    uint64_t todo = __ballot(mb != 0ull);
-                                    while (todo) {
-                                        int l[4];
-                                        bool lv[4];
+                                    if (lane == (unsigned)w) a.recMask[off + w] = hv[w];
+                                    if ((hv[w] >> lane) & 1ull) a.recMask[off + pre + popc_below64(hv[w])] = rv[w];
+                                    pre += (uint32_t)__popcll(hv[w]);
+                                }
+                                R.used += nzm;
+                                if (lane == 0) a.recDir[REC_DW * (dbase + chunkIt) + 2] = off + 1;
+                            }
+                        }
+                        R.want += nzm > (uint32_t)Cfg::MW ? nzm : 0u;
+                    } else if constexpr (!RP) {
+                        // reserve one range per candidate, then place; a
+                        // reservation past the candidate's row end is an
+                        // error and its hits are dropped
+                        for (uint32_t j = lane; j < m; j += 64) {
+                            const uint32_t v = L.cnt[j];
+                            if (v) {
+                                unsigned long long b = atomicAdd(&a.mirCur[c0 + j], (unsigned long long)v);
+                                if (b + v > a.offS[c0 + j + 1] - a.outBase) {
+                                    err++;
+                                    b = NO_SLOT;
+                                }
+                                L.base[j] = b;
+                            }
+                            L.cnt[j] = 0u;
+                        }
+                        wave_lds_sync();
+                        // place the mirrored hits, one source lane at a time
+                        if (__ballot(anyMir)) {
 #pragma unroll
-                                        for (int u = 0; u < 4; u++) {
-                                            lv[u] = todo != 0ull;
-                                            l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
-                                            todo &= todo - 1ull;
-                                        }
-                                        bool hb[4];
-                                        uint32_t j[4];
-                                        int32_t oid[4];
+                            for (int w = 0; w < Cfg::MW; w++) {
+                                const uint64_t mb = w == 0 ? mk0 : (w == 1 ? mk1 : (w == 2 ? mk2 : mk3));
+                                uint64_t todo = __ballot(mb != 0ull);
+                                while (todo) {
+                                    int l[4];
+                                    bool lv[4];
 #pragma unroll
-                                        for (int u = 0; u < 4; u++) {
-                                            const uint64_t bits = lv[u] ? readlane_u64(mb, l[u]) : 0ull;
-                                            hb[u] = (bits >> lane) & 1ull;
-                                            j[u] = min(readlane_u32(clo, l[u]) + (uint32_t)w * 64u + lane - c0,
-                                                       (uint32_t)(CH - 1));
-                                            oid[u] = (int32_t)readlane_u32((uint32_t)myId, l[u]);
-                                        }
-                                        uint32_t k2[4];
-                                        unsigned long long bb[4];
-#pragma unroll
-                                        for (int u = 0; u < 4; u++) {
-                                            k2[u] = hb[u] ? atomicAdd(&L.cnt[j[u]], 1u) : 0u;
-                                            bb[u] = L.base[j[u]];
-                                        }
-#pragma unroll
-                                        for (int u = 0; u < 4; u++)
-                                            if (hb[u] && bb[u] != NO_SLOT) a.ids[bb[u] + k2[u]] = oid[u];
+                                    for (int u = 0; u < 4; u++) {
+                                        lv[u] = todo != 0ull;
+                                        l[u] = lv[u] ? __builtin_ctzll(todo) : 0;
+                                        todo &= todo - 1ull;
                                     }
+                                    bool hb[4];
+                                    uint32_t j[4];
+                                    int32_t oid[4];
+#pragma unroll
+                                    for (int u = 0; u < 4; u++) {
+                                        const uint64_t bits = lv[u] ? readlane_u64(mb, l[u]) : 0ull;
+                                        hb[u] = (bits >> lane) & 1ull;
+                                        j[u] = min(readlane_u32(clo, l[u]) + (uint32_t)w * 64u + lane - c0,
+                                                   (uint32_t)(CH - 1));
+                                        oid[u] = (int32_t)readlane_u32((uint32_t)myId, l[u]);
+                                    }
+                                    uint32_t k2[4];
+                                    unsigned long long bb[4];
+#pragma unroll
+                                    for (int u = 0; u < 4; u++) {
+                                        k2[u] = hb[u] ? atomicAdd(&L.cnt[j[u]], 1u) : 0u;
+                                        bb[u] = L.base[j[u]];
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < 4; u++)
+                                        if (hb[u] && bb[u] != NO_SLOT) a.ids[bb[u] + k2[u]] = oid[u];
                                 }
                             }
                         }
                     }
-                    wave_lds_sync();
+                }
+                wave_lds_sync();
+            }
+        }
+    };
+
+    if constexpr (K < 4) {
+        // neighbour rows one after another: odometer over dims 1..K-1
+        int o[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) o[j] = -1;
+        o[0] = 0;
+        for (int it = 0; it < nrow; ++it) {
+            bool inb = true, originRow = true;
+            int64_t dl = 0;
+#pragma unroll
+            for (int j = 1; j < K; j++) {
+                originRow = originRow && o[j] == 0;
+                const int64_t v = (int64_t)c[j] + o[j];
+                inb = inb && v >= 0 && v < (int64_t)a.nCells[j];
+                dl += (int64_t)o[j] * a.mult[j];
+            }
+            if (inb) row(o, (uint64_t)((int64_t)rowBase + dl), originRow, false, 0u, 0u);
+            bool carry = true;
+#pragma unroll
+            for (int j = 1; j < K; j++) {
+                if (carry) {
+                    if (o[j] == 1) {
+                        o[j] = -1;
+                    } else {
+                        o[j]++;
+                        carry = false;
+                    }
                 }
             }
         }
-        bool carry = true;
+    } else {
+        // 3^(K-1) neighbour rows, 64 at a time: lane l looks up row b0 + l
+        // (its bounds and whether it can hold claimed candidates), then the
+        // wave visits the rows that can, in the same order as the odometer
+        // (o[1] fastest).  Rows left out are outside the grid, empty, or
+        // claimed by no lane: LID claims a row with every lane and cell iff
+        // its offset dl > 0 (|dl| >= nCells[0] off the origin row), Unicomp
+        // off the origin row by the parity of c[j] of its highest non-zero
+        // offset, the same for every lane.
+        for (int b0 = 0; b0 < nrow; b0 += 64) {
+            const int my = b0 + (int)lane;
+            int od[K];
+            od[0] = 0;
+            bool inb = my < nrow, org = true;
+            int64_t dl = 0;
+            int t = my;
 #pragma unroll
-        for (int j = 1; j < K; j++) {
-            if (carry) {
-                if (o[j] == 1) {
-                    o[j] = -1;
-                } else {
-                    o[j]++;
-                    carry = false;
+            for (int j = 1; j < K; j++) {
+                od[j] = t % 3 - 1;
+                t /= 3;
+                org = org && od[j] == 0;
+                const int64_t v = (int64_t)c[j] + od[j];
+                inb = inb && v >= 0 && v < (int64_t)a.nCells[j];
+                dl += (int64_t)od[j] * a.mult[j];
+            }
+            uint32_t r0 = 0u, r1 = 0u;
+            bool act = false;
+            if (inb) {
+                const uint64_t rbl = (uint64_t)((int64_t)rowBase + dl);
+                bool may = true;
+                if (a.form == SJ_FORM_LID) may = org || dl > 0;
+                else if (a.form == SJ_FORM_UNICOMP && !org) may = claims<K>(a.form, c, od, linA, rbl);
+                if (may) {
+                    r0 = pos_lb(a, rbl + ux0);
+                    r1 = pos_lb(a, rbl + ux1 + 1);
+                    act = r1 > r0;
                 }
+            }
+            uint64_t am = __ballot(act);
+            while (am) {
+                const int bl = __builtin_ctzll(am);
+                am &= am - 1ull;
+                int o[K];
+                o[0] = 0;
+                bool originRow = true;
+                int64_t dlr = 0;
+                int tt = b0 + bl;
+#pragma unroll
+                for (int j = 1; j < K; j++) {
+                    o[j] = tt % 3 - 1;
+                    tt /= 3;
+                    originRow = originRow && o[j] == 0;
+                    dlr += (int64_t)o[j] * a.mult[j];
+                }
+                row(o, (uint64_t)((int64_t)rowBase + dlr), originRow, true, readlane_u32(r0, bl),
+                    readlane_u32(r1, bl));
             }
         }
     }
