@@ -19,9 +19,11 @@ One step = grid construction + count pass + offsets + fill pass over points
 already resident in HBM, producing the exact neighbour table in HBM (for
 many64: one replay of the recorded graph over the planned grids).  Prints
 one JSON line.  roofline.achieved / frac: the join's algorithmic bytes
-N*d*8 + |S|*4 + (N+1)*8 (FP32 matrix-core flops 2*d*C for clusters64d) over
-the step time; kernel_achieved / kernel_frac: the same work over the
-dominant kernel's time, from HIP events on the stream it runs on.
+N*d*8 + |S|*4 + (N+1)*8 over the step time (clusters64d: FP32 matrix-core
+flops 2*d*C; uniform6d and many64: the exact predicate's (3d-1) flops per
+claimed candidate against the vector FP64 peak, bytes under roofline.hbm);
+kernel_achieved / kernel_frac: the same work over the dominant kernel's
+time, from HIP events on the stream it runs on.
 cpu_baseline is the CPU oracle (grid join, OpenMP) on the host cores; the
 oracle's table is also the reference of config.exact_vs_oracle (an
 order-free fingerprint of every GPU table), and a mismatch exits 1.
@@ -39,6 +41,14 @@ sys.path.insert(0, os.path.join(ROOT, "self-join-hpbdc2019_amd"))
 
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
+# vector FP64 peak (FMA = 2 flops); unpacked FP32 VALU issues at the same rate
+VALU_FP64_PEAK_TFLOPS = 78.6
+
+
+def predicate_flops(candidates, d):
+    """Flops of the exact predicate over `candidates` pairs: d differences,
+    one square and d - 1 fused multiply-adds per pair."""
+    return float(candidates) * (3 * d - 1)
 METRIC = "result pairs/sec & join time at 2M pts, FP32+recheck vs FP64, % HBM roofline"
 
 SINGLE = {
@@ -279,6 +289,19 @@ def bench_single(a, torch):
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes": balg, "step_ms": ms,
                 "kernel": kname, "kernel_ms": kms, "kernel_achieved": kach, "kernel_frac": kach / HBM_PEAK_GBS}
+        if a.workload == "uniform6d":
+            # the 6-D walk is bound by distance arithmetic, not bytes: the
+            # predicate's flops over the claimed candidates, against the
+            # vector FP64 peak, for the step and for the count pass
+            flops = predicate_flops(st["candidates"], d)
+            hbm = roof
+            roof = {"bound": "valu", "achieved": flops / dt / 1e12, "peak": VALU_FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": flops / dt / 1e12 / VALU_FP64_PEAK_TFLOPS, "traffic": traffic,
+                    "alg_flops": flops, "candidates": st["candidates"], "step_ms": ms,
+                    "kernel": "count", "kernel_ms": st["ms_count"],
+                    "kernel_achieved": flops / (st["ms_count"] * 1e-3) / 1e12,
+                    "kernel_frac": flops / (st["ms_count"] * 1e-3) / 1e12 / VALU_FP64_PEAK_TFLOPS,
+                    "candidates_per_s": st["candidates"] / dt, "hbm": hbm}
     out = {
         "metric": METRIC,
         "value": total / dt,
@@ -363,6 +386,7 @@ def bench_many(a, torch):
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / max(a.steps, 1)
             st = m.stats()
+            flops = sum(predicate_flops(m.stats(i)["candidates"], batch[i][1]) for i in range(len(batch)))
             fps = []
             for i in range(len(batch)):
                 o, ids, rid, tot = m.device_table(i)
@@ -371,7 +395,7 @@ def bench_many(a, torch):
                                        sjamd.device_copy(ids, tot, torch.int32),
                                        sjamd.device_copy(rid, n, torch.int32), torch))
             res[name] = dict(dt=dt, dev_ms=dev / max(a.steps, 1), total=st["total_pairs"], stats=st,
-                             times=m.times(), tuning=tuned, fps=fps)
+                             times=m.times(), tuning=tuned, fps=fps, flops=flops)
             m.free()
     f32 = res["fp32"]
     total = f32["total"]
@@ -410,11 +434,18 @@ def bench_many(a, torch):
                          "misclassified_before_recheck": f32["stats"]["fp32_misclassified"],
                          "candidates": f32["stats"]["candidates"]},
         "fp32_speedup_vs_fp64": res["fp64"]["dt"] / f32["dt"],
-        "roofline": {"bound": "hbm", "achieved": kbytes / f32["dt"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kbytes / f32["dt"] / 1e9 / HBM_PEAK_GBS, "traffic": None, "alg_bytes": kbytes,
-                     "step_ms": f32["dt"] * 1e3,
+        # bound by distance arithmetic: the predicate's flops over every
+        # instance's claimed candidates against the vector FP64 peak
+        "roofline": {"bound": "valu", "achieved": f32["flops"] / f32["dt"] / 1e12, "peak": VALU_FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": f32["flops"] / f32["dt"] / 1e12 / VALU_FP64_PEAK_TFLOPS,
+                     "traffic": None, "alg_flops": f32["flops"], "step_ms": f32["dt"] * 1e3,
                      "kernel": "graph replay (count + offsets + fill, all instances)", "kernel_ms": kms,
-                     "kernel_achieved": achieved, "kernel_frac": achieved / HBM_PEAK_GBS},
+                     "kernel_achieved": f32["flops"] / (kms * 1e-3) / 1e12,
+                     "kernel_frac": f32["flops"] / (kms * 1e-3) / 1e12 / VALU_FP64_PEAK_TFLOPS,
+                     "candidates_per_s": f32["stats"]["candidates"] / f32["dt"],
+                     "hbm": {"bound": "hbm", "achieved": kbytes / f32["dt"] / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": kbytes / f32["dt"] / 1e9 / HBM_PEAK_GBS, "alg_bytes": kbytes,
+                             "kernel_achieved": achieved, "kernel_frac": achieved / HBM_PEAK_GBS}},
     }
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -390,10 +390,27 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
 #pragma unroll
     for (int j = 1; j < K; j++) nrow *= 3;
 
+    // candidates [c0, c0 + m) into LDS rows [0, m)
+    auto stage = [&](uint32_t c0, uint32_t m) __attribute__((always_inline)) {
+        for (uint32_t j = lane; j < m; j += 64) {
+            const uint64_t t = (uint64_t)c0 + j;
+            if constexpr (RP) {
+            } else if constexpr (PREC == 1) {
+#pragma unroll
+                for (int q = 0; q < DD; q++) L.x32[j][q] = (float)(a.pts[t * DD + q] - ctr[q]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
+            }
+            if (PASS == PASS_FILL) L.id[j] = (int32_t)a.origId[t];  // emission only
+            L.cnt[j] = 0u;
+        }
+    };
+
     // One neighbour row at offsets o[1..K-1] (o[0] unused) with cell-row
-    // base rb; the caller may pass the union's bounds [pu0, pu1) (pre).
+    // base rb; the caller may pass the union's bounds [pu0, pu1) (given).
     // Every lane shares the row coordinates, so control flow is uniform.
-    auto row = [&](const int (&o)[K], uint64_t rb, bool originRow, bool pre, uint32_t pu0,
+    auto row = [&](const int (&o)[K], uint64_t rb, bool originRow, bool given, uint32_t pu0,
                    uint32_t pu1) __attribute__((always_inline)) {
         // claimed dim-0 offsets of this lane's cell in this row
         bool useL = false, useC = false, useR = false;
@@ -409,10 +426,30 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
             useC = originRow || claims<K>(a.form, c, oo, linA, rb + x);
         }
         if (__ballot(useL || useC || useR)) {
-            const uint32_t u0 = pre ? pu0 : pos_lb(a, rb + ux0);
-            const uint32_t u1 = pre ? pu1 : pos_lb(a, rb + ux1 + 1);
+            const uint32_t u0 = given ? pu0 : pos_lb(a, rb + ux0);
+            const uint32_t u1 = given ? pu1 : pos_lb(a, rb + ux1 + 1);
             uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
-            if (valid) {
+            // given: the first chunk is staged while the window bounds load
+            // (both requests in flight together); the lookups run on every
+            // lane, clamped to the row, and past-the-end lanes drop them
+            bool staged0 = false;
+            if (given) {
+                const uint32_t xl = x >= 1 ? x - 1 : x;
+                const uint32_t xr = x + 1 < nC0 ? x + 2 : x + 1;
+                const uint32_t f0 = pos_lb(a, rb + xl), f1 = pos_lb(a, rb + x);
+                const uint32_t f2 = pos_lb(a, rb + x + 1), f3 = pos_lb(a, rb + xr);
+                if constexpr (!RP) {
+                    const uint32_t m0 = min((uint32_t)CH, u1 - u0);
+                    if (!(PASS == PASS_FILL && !anyOwnW && (u0 + m0 <= a.wBegin || u0 >= a.wEnd))) {
+                        stage(u0, m0);
+                        staged0 = true;
+                    }
+                }
+                e0 = valid ? f0 : 0u;
+                e1 = valid ? f1 : 0u;
+                e2 = valid ? f2 : 0u;
+                e3 = valid ? f3 : 0u;
+            } else if (valid) {
                 e1 = pos_lb(a, rb + x);
                 e2 = pos_lb(a, rb + x + 1);
                 e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
@@ -488,20 +525,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                         }
                     }
                 }
-                // stage the chunk
-                for (uint32_t j = lane; j < m; j += 64) {
-                    const uint64_t t = (uint64_t)c0 + j;
-                    if constexpr (RP) {
-                    } else if constexpr (PREC == 1) {
-#pragma unroll
-                        for (int q = 0; q < DD; q++) L.x32[j][q] = (float)(a.pts[t * DD + q] - ctr[q]);
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < DD; q++) L.x64[j][q] = a.pts[t * DD + q];
-                    }
-                    if (PASS == PASS_FILL) L.id[j] = (int32_t)a.origId[t];  // emission only
-                    L.cnt[j] = 0u;
-                }
+                if (!(staged0 && c0 == u0)) stage(c0, m);
                 wave_lds_sync();
                 // mirrored hits of the chunk, one word per 64-candidate piece
                 uint64_t mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
