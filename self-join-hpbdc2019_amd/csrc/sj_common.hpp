@@ -185,6 +185,15 @@ struct sj_context_s {
     // neighbour IDs of the largest result seen (plus a margin): a join whose
     // result fits fills without first reading |S| back to the host
     uint64_t idsHint = 0;
+    // key width of the last grid built on this context, with its shape and
+    // eps: a build of the same shape sorts with it before the grid
+    // parameters reach the host, and sorts again if they need more bits
+    struct {
+        bool valid = false, wide = false;
+        int d = 0, k = 0, bits = 0;
+        uint64_t n = 0;
+        double eps = 0.0;
+    } gridHint;
 };
 
 struct sj_grid_s {

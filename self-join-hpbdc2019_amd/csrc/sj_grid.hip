@@ -314,7 +314,7 @@ static int bits_for(uint64_t maxv) {
 }
 
 template <class KT>
-static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t *d_flags) {
+static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t *d_flags, int bits) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t n = g->n;
@@ -329,7 +329,6 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
     SJ_LAUNCH_CHECK();
     KT *kp = keys.p, *ka = keys2.p;
     uint32_t *vp = vals.p, *va = vals2.p;
-    const int bits = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
     sj_status st;
     if constexpr (sizeof(KT) == 8)
         st = radix_sort_pairs_u64(pool, s, &kp, &ka, &vp, &va, n, bits);
@@ -390,29 +389,60 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
                        flags.p);
     SJ_LAUNCH_CHECK();
-    SJ_TRY_HIP(hipMemcpyAsync(&g->gp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
-    SJ_TRY_HIP(hipStreamSynchronize(s));
-    if (g->gp.flags & GP_NONFINITE) {
-        set_error("grid_build: non-finite coordinate");
-        return SJ_ERR_INVALID;
+    // the parameters land in pinned words 16.. (the flags use word 0)
+    GridParams *pgp = reinterpret_cast<GridParams *>(ctx->pinned + 16);
+    static_assert(16 * 8 + sizeof(GridParams) <= 64 * 8, "pinned readback space");
+    SJ_TRY_HIP(hipMemcpyAsync(pgp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
+    auto check_params = [&]() -> sj_status {
+        g->gp = *pgp;
+        if (g->gp.flags & GP_NONFINITE) {
+            set_error("grid_build: non-finite coordinate");
+            return SJ_ERR_INVALID;
+        }
+        if (g->gp.flags & GP_OVERFLOW) {
+            set_error("grid_build: cell count overflows (nCells >= 2^32 or totalCells >= 2^64)");
+            return SJ_ERR_OVERFLOW;
+        }
+        return SJ_OK;
+    };
+    // Same shape and eps as the context's last grid: sort with its key
+    // width now and check the parameters when the host next waits (the
+    // keys are computed from the device parameters either way; a wider
+    // grid than guessed is sorted again).  Otherwise wait for them first.
+    const auto &h = ctx->gridHint;
+    const bool guess = h.valid && h.n == n && h.d == d && h.k == k && h.eps == eps;
+    int bits = h.bits;
+    bool wide = h.wide;
+    if (!guess) {
+        SJ_TRY_HIP(hipStreamSynchronize(s));
+        SJ_TRY(check_params());
+        bits = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
+        wide = g->gp.totalCells > 0xFFFFFFFFull;
     }
-    if (g->gp.flags & GP_OVERFLOW) {
-        set_error("grid_build: cell count overflows (nCells >= 2^32 or totalCells >= 2^64)");
-        return SJ_ERR_OVERFLOW;
+    for (int attempt = 0;; attempt++) {
+        if (wide)
+            SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags.p, bits));
+        else
+            SJ_TRY(sort_and_compact<uint32_t>(g, d_points, flags.p, bits));
+        // the coordinates in cell order need no host value: enqueued before
+        // the host waits for the flags and the non-empty cell count
+        SJ_TRY(g->sortedPts.alloc(pool, n * (uint64_t)d));
+        hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n * (uint64_t)d)), dim3(GB), 0, s, d_points,
+                           g->sortedIds.p, n, d, g->sortedPts.p);
+        SJ_LAUNCH_CHECK();
+        SJ_TRY_HIP(hipEventSynchronize(ctx->ev));
+        if (guess && attempt == 0) {
+            SJ_TRY(check_params());  // copied before the event
+            const int need = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
+            const bool needWide = g->gp.totalCells > 0xFFFFFFFFull;
+            if (need > bits || (needWide && !wide)) {
+                bits = need;
+                wide = needWide;
+                continue;
+            }
+        }
+        break;
     }
-
-    if (g->gp.totalCells <= 0xFFFFFFFFull)
-        SJ_TRY(sort_and_compact<uint32_t>(g, d_points, flags.p));
-    else
-        SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags.p));
-
-    // the coordinates in cell order need no host value: enqueued before the
-    // host waits for the flags and the non-empty cell count
-    SJ_TRY(g->sortedPts.alloc(pool, n * (uint64_t)d));
-    hipLaunchKernelGGL(k_gather_points, dim3(blocks_for(n * (uint64_t)d)), dim3(GB), 0, s, d_points,
-                       g->sortedIds.p, n, d, g->sortedPts.p);
-    SJ_LAUNCH_CHECK();
-    SJ_TRY_HIP(hipEventSynchronize(ctx->ev));
     {
         const uint64_t v = ctx->pinned[0];
         if ((uint32_t)(v & 0xffffffffu) & GP_OVERFLOW) {
@@ -421,6 +451,13 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
         }
         g->nNonEmpty = (uint32_t)(v >> 32);
     }
+    ctx->gridHint.valid = true;
+    ctx->gridHint.wide = g->gp.totalCells > 0xFFFFFFFFull;
+    ctx->gridHint.d = d;
+    ctx->gridHint.k = k;
+    ctx->gridHint.bits = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
+    ctx->gridHint.n = n;
+    ctx->gridHint.eps = eps;
     const uint64_t nne = g->nNonEmpty;
     SJ_TRY(g->cellCoord.alloc(pool, nne * (uint64_t)k));
     hipLaunchKernelGGL(k_cell_coords, dim3(blocks_for(nne)), dim3(GB), 0, s, g->sortedPts.p,

@@ -171,6 +171,34 @@ def test_batched_rounds(ctx, form):
         np.testing.assert_array_equal(got, want)
 
 
+def test_grid_key_width_from_last_build(ctx):
+    """A grid of the same shape and eps as the context's previous one is
+    sorted with that grid's key width before its own parameters reach the
+    host; a grid that needs more bits (or 64-bit keys) is sorted again.
+    Each grid and join below must equal the oracle's whatever the guess."""
+    rng = np.random.default_rng(21)
+    base = rng.random((3000, 3))
+    eps = 0.05
+    # ~14-bit keys, then 64-bit keys (totalCells > 2^32), then ~21 bits
+    # (a wider guess than needed), then the first set again
+    for scale in (1.0, 100.0, 6.0, 1.0):
+        pts = np.ascontiguousarray(base * scale)
+        og = O.OracleGrid(pts, 3, eps)
+        _say(f"[grid guess] scale={scale} totalCells={og.totalCells}")
+        g = ctx.grid_build(pts, 3, eps)
+        mn, mx, nc, tot = g.dims()
+        assert tot == og.totalCells
+        np.testing.assert_array_equal(nc, og.nCells)
+        a = g.arrays()
+        np.testing.assert_array_equal(a["cellLinear"], og.cellLinear)
+        np.testing.assert_array_equal(a["indexLookup"], og.indexLookup)
+        np.testing.assert_array_equal(a["indexMin"], og.indexMin)
+        r = g.join(form="lid")
+        _assert_same(r.csr(), og.join(O.LID, sort_rows=True))
+        r.free()
+        g.free()
+
+
 def test_edge_single_point(ctx):
     _say("[edge] single point")
     r = ctx.self_join(np.array([[0.25, 0.75]]), 2, 0.1)
