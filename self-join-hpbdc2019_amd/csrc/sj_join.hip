@@ -468,9 +468,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                 const uint32_t clo = max(lo, c0);
                 const uint32_t chi = min(hi, c0 + m);
                 const uint32_t len = chi > clo ? chi - clo : 0u;
-                uint32_t maxLen = len;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) maxLen = max(maxLen, (uint32_t)__shfl_xor(maxLen, off, 64));
+                static_assert(CH <= 256, "lane spans need at most 9 bits");
+                const uint32_t maxLen = wave_max_small<CH == 256 ? 9 : 8>(len);
                 // replay: the chunk's recorded masks, requested before the
                 // staging so that both loads are in flight together
                 uint64_t rm[Cfg::MW];

@@ -108,13 +108,27 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// Bits i in [0,64) with start + i in [A, B).
+// Bits i in [0,64) with start + i in [A, B).  Branch-free: both ends
+// clamped to [0, 64] with unsigned differences, then one run of w bits.
 __device__ __forceinline__ uint64_t range_bits(uint32_t start, uint32_t A, uint32_t B) {
-    const int64_t lo = max((int64_t)A - (int64_t)start, (int64_t)0);
-    const int64_t hi = min((int64_t)B - (int64_t)start, (int64_t)64);
-    if (hi <= lo) return 0ull;
-    const uint64_t w = (uint64_t)(hi - lo);
-    return (w >= 64 ? ~0ull : ((1ull << w) - 1ull)) << lo;
+    const uint32_t lo = min(A > start ? A - start : 0u, 64u);
+    const uint32_t hi = min(B > start ? B - start : 0u, 64u);
+    const uint32_t w = hi > lo ? hi - lo : 0u;
+    return w ? (~0ull >> (64u - w)) << lo : 0ull;  // w > 0: lo < 64
+}
+
+// Largest v over the wave's active lanes, for v < 2^BITS: the bits of the
+// result from the top down, each kept when some lane reaches it (ballots
+// only, no cross-lane data movement).
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_max_small(uint32_t v) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = BITS - 1; b >= 0; b--) {
+        const uint32_t t = r | (1u << b);
+        if (__ballot(v >= t)) r = t;
+    }
+    return r;
 }
 
 // Value of v in lane l (l wave-uniform).
