@@ -222,14 +222,23 @@ struct TileLds {
     unsigned long long base[C::CH];
 };
 
+// Own hits of one piece staged in output order by the replaying fill (at
+// most OWN_STAGE), then stored 64 consecutive entries at a time.
+constexpr uint32_t OWN_STAGE = 384;
+
 // LDS of the fill that replays a hit record: candidate IDs and mirrored-hit
-// counters only, chunked like the count pass of precision PREC.
+// counters, chunked like the count pass of precision PREC, and the own-hit
+// stage: entry i is neighbour ID sid[i] for position wc[l] + t of lane l's
+// row, with sdst[i] = l * 64 + t.
 template <int D, int PREC>
 struct TileLdsIds {
     using C = TileCfg<D, PREC>;
     int32_t id[C::CH];
     uint32_t cnt[C::CH];
     unsigned long long base[C::CH];
+    unsigned long long wc[64];
+    int32_t sid[OWN_STAGE];
+    uint32_t sdst[OWN_STAGE];
 };
 
 // Upper bound of the pieces a tile walk executes: per neighbour row in the
@@ -644,12 +653,46 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     // one source lane at a time, its mask broadcast so that
                     // each thread takes one candidate bit and the hits
                     // leave as one contiguous row write
-                    uint32_t mp = PASS == PASS_FILL ? (uint32_t)__popcll(ownBits) : 0u;
-                    if (PASS == PASS_FILL) {
+                    const uint32_t cntOwn = PASS == PASS_FILL ? (uint32_t)__popcll(ownBits) : 0u;
+                    const uint32_t mp = PASS == PASS_FILL ? wave_max_small<7>(cntOwn) : 0u;
+                    bool staged = false;
+                    if constexpr (RP) {
+                        // replay: the piece's own hits (when few per lane and
+                        // in total) go through the LDS stage in output order,
+                        // so that each store covers a few row segments
+                        // instead of one position in each of 64 rows
+                        if (mp <= 24u) {
+                            uint32_t pre = 0, tot = 0;
 #pragma unroll
-                        for (int off = 32; off > 0; off >>= 1) mp = max(mp, (uint32_t)__shfl_xor(mp, off, 64));
+                            for (int b = 0; b < 7; b++) {
+                                const uint64_t bm = __ballot((cntOwn >> b) & 1u);
+                                pre += popc_below64(bm) << b;
+                                tot += (uint32_t)__popcll(bm) << b;
+                            }
+                            if (tot <= OWN_STAGE) {
+                                if (cntOwn) L.wc[lane] = wcur;
+                                uint64_t ob = ownBits;
+                                uint32_t q = pre, t = lane * 64u;
+                                while (ob) {
+                                    const int bi = __builtin_ctzll(ob);
+                                    ob &= ob - 1ull;
+                                    L.sid[q] = L.id[min(jb + (uint32_t)bi, (uint32_t)(CH - 1))];
+                                    L.sdst[q] = t;
+                                    q++;
+                                    t++;
+                                }
+                                wave_lds_sync();
+                                for (uint32_t i = lane; i < tot; i += 64) {
+                                    const uint32_t pk = L.sdst[i];
+                                    a.ids[L.wc[pk >> 6] + (pk & 63u)] = L.sid[i];
+                                }
+                                wave_lds_sync();
+                                staged = true;
+                            }
+                        }
                     }
-                    if (PASS == PASS_FILL && mp <= 24u) {
+                    if (staged) {
+                    } else if (PASS == PASS_FILL && mp <= 24u) {
                         uint64_t ob = ownBits;
                         uint64_t wp = wcur;
                         while (ob) {
@@ -992,8 +1035,10 @@ struct cmax {
 };
 template <int D>
 struct LdsMax {
-    static constexpr int v = cmax<cmax<(int)sizeof(TileLds<D, 0>), (int)sizeof(TileLds<D, 1>)>::v,
-                                  LdsMax<D - 1>::v>::v;
+    static constexpr int v =
+        cmax<cmax<cmax<(int)sizeof(TileLds<D, 0>), (int)sizeof(TileLds<D, 1>)>::v,
+                  cmax<(int)sizeof(TileLdsIds<D, 0>), (int)sizeof(TileLdsIds<D, 1>)>::v>::v,
+             LdsMax<D - 1>::v>::v;
 };
 template <>
 struct LdsMax<0> {
