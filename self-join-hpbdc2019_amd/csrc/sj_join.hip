@@ -206,7 +206,7 @@ __device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s, Lan
 template <int D, int PREC>
 struct TileCfg {
     static constexpr int DD = D > 0 ? D : 1;
-    static constexpr int CH = (PREC == 1 || D <= 4) ? 256 : 128;  // staged candidates per chunk
+    static constexpr int CH = D <= 4 ? 256 : 128;  // staged candidates per chunk
     static constexpr int MW = CH / 64;                              // mask words per lane (<= 4)
 };
 
