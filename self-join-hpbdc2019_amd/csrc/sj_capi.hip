@@ -5,6 +5,8 @@
 #include <cstring>
 #include <new>
 
+#include <thread>
+
 #include "sj_common.hpp"
 #include "sj_traverse.hpp"
 
@@ -34,15 +36,18 @@ struct ResultHostCache {
 };
 
 // device-resident table downloaded as one host batch (rows in cell order)
+// the device table as one host batch: three downloads into pinned memory
+// from the context's pool, queued together on the context's stream
 sj_status device_as_batch(sj_result r, sj_batch_host &B) {
     const uint64_t n = r->n;
-    B.rowOffsets.resize(n + 1);
-    B.rowIds.resize(n);
+    hipStream_t s = r->ctx->stream;
+    SJ_TRY(B.rowOffsets.alloc(&r->ctx->hpool, n + 1));
+    SJ_TRY(B.rowIds.alloc(&r->ctx->hpool, n));
     SJ_TRY(B.ids.alloc(&r->ctx->hpool, r->total));
-    SJ_TRY_HIP(hipStreamSynchronize(r->ctx->stream));
-    SJ_TRY_HIP(hipMemcpy(B.rowOffsets.data(), r->d_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost));
-    SJ_TRY_HIP(hipMemcpy(B.rowIds.data(), r->d_rowIds.p, n * 4, hipMemcpyDeviceToHost));
-    if (r->total) SJ_TRY_HIP(hipMemcpy(B.ids.p, r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost));
+    SJ_TRY_HIP(hipMemcpyAsync(B.rowOffsets.p, r->d_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipMemcpyAsync(B.rowIds.p, r->d_rowIds.p, n * 4, hipMemcpyDeviceToHost, s));
+    if (r->total) SJ_TRY_HIP(hipMemcpyAsync(B.ids.p, r->d_ids.p, r->total * 4, hipMemcpyDeviceToHost, s));
+    SJ_TRY_HIP(hipStreamSynchronize(s));
     return SJ_OK;
 }
 
@@ -59,7 +64,7 @@ sj_status ensure_host_csr(sj_result r, std::vector<uint64_t> &off, std::vector<i
     }
     std::vector<uint64_t> cnt(r->n, 0);
     for (auto &B : r->batches)
-        for (size_t q = 0; q < B.rowIds.size(); q++) cnt[B.rowIds[q]] = B.rowOffsets[q + 1] - B.rowOffsets[q];
+        for (size_t q = 0; q < B.rowIds.n; q++) cnt[B.rowIds.p[q]] = B.rowOffsets.p[q + 1] - B.rowOffsets.p[q];
     uint64_t s = 0;
     for (uint64_t i = 0; i < r->n; i++) {
         off[i] = s;
@@ -67,8 +72,8 @@ sj_status ensure_host_csr(sj_result r, std::vector<uint64_t> &off, std::vector<i
     }
     off[r->n] = s;
     for (auto &B : r->batches)
-        for (size_t q = 0; q < B.rowIds.size(); q++)
-            std::copy(B.ids.p + B.rowOffsets[q], B.ids.p + B.rowOffsets[q + 1], ids.begin() + off[B.rowIds[q]]);
+        for (size_t q = 0; q < B.rowIds.n; q++)
+            std::copy(B.ids.p + B.rowOffsets.p[q], B.ids.p + B.rowOffsets.p[q + 1], ids.begin() + off[B.rowIds.p[q]]);
     return SJ_OK;
 }
 
@@ -379,20 +384,38 @@ SJ_API sj_status sj_result_neighbor_table(sj_result r, sj_neighbor_lookup *table
         set_error("sj_result_neighbor_table: null argument");
         return SJ_ERR_INVALID;
     }
-    for (uint64_t p = 0; p < r->n; p++) table[p] = {(int32_t)p, 0, -1, -1};
     if (r->on_device && r->batches.empty()) {
         r->batches.resize(1);
         SJ_TRY(device_as_batch(r, r->batches[0]));
     }
+    uint64_t rows = 0;
     for (size_t b = 0; b < r->batches.size(); b++) {
-        const auto &B = r->batches[b];
-        if (B.ids.n > 0x7fffffffull) {
+        if (r->batches[b].ids.n > 0x7fffffffull) {
             set_error("neighbor table: batch above 2^31-1 pairs");
             return SJ_ERR_OVERFLOW;
         }
-        for (size_t q = 0; q < B.rowIds.size(); q++)
-            table[B.rowIds[q]] = {(int32_t)B.rowIds[q], (int32_t)B.rowOffsets[q],
-                                  (int32_t)B.rowOffsets[q + 1] - 1, (int32_t)b};
+        rows += r->batches[b].rowIds.n;
+    }
+    // rows of no batch (none when the batches cover every point): empty
+    if (rows != r->n)
+        for (uint64_t p = 0; p < r->n; p++) table[p] = {(int32_t)p, 0, -1, -1};
+    // each batch's rows, split over a few host threads for large batches
+    for (size_t b = 0; b < r->batches.size(); b++) {
+        const auto &B = r->batches[b];
+        auto part = [&](uint64_t q0, uint64_t q1) {
+            for (uint64_t q = q0; q < q1; q++)
+                table[B.rowIds.p[q]] = {(int32_t)B.rowIds.p[q], (int32_t)B.rowOffsets.p[q],
+                                        (int32_t)B.rowOffsets.p[q + 1] - 1, (int32_t)b};
+        };
+        const uint64_t m = B.rowIds.n;
+        const unsigned nt = m >= (1u << 20) ? 8u : 1u;
+        if (nt == 1) {
+            part(0, m);
+            continue;
+        }
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++) th.emplace_back(part, m * t / nt, m * (t + 1) / nt);
+        for (auto &t : th) t.join();
     }
     return SJ_OK;
 }

@@ -221,9 +221,9 @@ struct sj_grid_s {
 };
 
 struct sj_batch_host {
-    sj::DBuf<int32_t> ids;             // neighbour IDs of the batch rows (pinned host)
-    std::vector<uint32_t> rowIds;      // original IDs of the rows
-    std::vector<uint64_t> rowOffsets;  // rows+1, into ids
+    sj::DBuf<int32_t> ids;            // neighbour IDs of the batch rows (pinned host)
+    sj::DBuf<uint32_t> rowIds;        // original IDs of the rows (pinned host; .n rows)
+    sj::DBuf<uint64_t> rowOffsets;    // rows+1, into ids (pinned host)
 };
 
 struct sj_result_s {

@@ -2368,9 +2368,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             const uint64_t cnt = hOff.p[s1] - base;
             sj_batch_host &B = r->batches[bi];
             SJ_TRY(B.ids.alloc(hp, cnt));
-            B.rowIds.assign(hIds.p + s0, hIds.p + s1);
-            B.rowOffsets.resize((size_t)(s1 - s0) + 1);
-            for (uint32_t q = s0; q <= s1; q++) B.rowOffsets[q - s0] = hOff.p[q] - base;
+            SJ_TRY(B.rowIds.alloc(hp, (size_t)(s1 - s0)));
+            SJ_TRY(B.rowOffsets.alloc(hp, (size_t)(s1 - s0) + 1));
+            std::copy(hIds.p + s0, hIds.p + s1, B.rowIds.p);
+            for (uint32_t q = s0; q <= s1; q++) B.rowOffsets.p[q - s0] = hOff.p[q] - base;
             SJ_TRY_HIP(hipStreamWaitEvent(cs, filled[sl].e, 0));
             SJ_TRY_HIP(hipMemcpyAsync(B.ids.p, dbuf[sl].p, cnt * 4, hipMemcpyDeviceToHost, cs));
             SJ_TRY_HIP(hipEventRecord(copied[sl].e, cs));

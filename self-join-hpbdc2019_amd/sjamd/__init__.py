@@ -458,7 +458,7 @@ class Result:
         return k[:self.total], v[:self.total]
 
     def neighbor_table(self):
-        tab = np.zeros((self.n, 4), dtype=np.int32)
+        tab = np.empty((self.n, 4), dtype=np.int32)  # every entry is written
         _check(lib().sj_result_neighbor_table(self.h, _np_ptr(tab)))
         return tab
 
