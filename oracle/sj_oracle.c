@@ -463,7 +463,8 @@ int sjo_join(const sjo_grid *g, const double *pts, int form, int nthreads,
     return SJO_OK;
 }
 
-/* Brute force with the same predicate (survey cross-check).  Rows sorted. */
+/* Brute force over all pairs with the same predicate: an independent check
+   of the grid joins on small inputs.  Rows sorted. */
 int sjo_brute(const double *pts, uint64_t n, int d, double eps, int nthreads,
               uint64_t *offsets, int32_t **ids_out)
 {

@@ -182,7 +182,8 @@ def test_claims_exactly_once(form, k):
 
 
 def test_unicomp_visit_counts_2d_and_3d():
-    # survey enumeration: Unicomp visits 0..3^k-1 neighbour cells
+    # exhaustive enumeration: a Unicomp cell visits between 0 and 3^k-1
+    # of its neighbour cells
     for k in (2, 3):
         nc = np.full(k, 8, dtype=np.uint32)
         counts = set()
