@@ -903,24 +903,26 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     act = r1 > r0;
                 }
             }
+            // the visited rows' offsets (2 bits per dim) and bases come from
+            // the lane that looked the row up
+            uint32_t opk = 0;
+#pragma unroll
+            for (int j = 1; j < K; j++) opk |= (uint32_t)(od[j] + 1) << (2 * (j - 1));
+            const uint64_t rbl = (uint64_t)((int64_t)rowBase + dl);
             uint64_t am = __ballot(act);
             while (am) {
                 const int bl = __builtin_ctzll(am);
                 am &= am - 1ull;
+                const uint32_t ok = readlane_u32(opk, bl);
                 int o[K];
                 o[0] = 0;
                 bool originRow = true;
-                int64_t dlr = 0;
-                int tt = b0 + bl;
 #pragma unroll
                 for (int j = 1; j < K; j++) {
-                    o[j] = tt % 3 - 1;
-                    tt /= 3;
+                    o[j] = (int)((ok >> (2 * (j - 1))) & 3u) - 1;
                     originRow = originRow && o[j] == 0;
-                    dlr += (int64_t)o[j] * a.mult[j];
                 }
-                row(o, (uint64_t)((int64_t)rowBase + dlr), originRow, true, readlane_u32(r0, bl),
-                    readlane_u32(r1, bl));
+                row(o, readlane_u64(rbl, bl), originRow, true, readlane_u32(r0, bl), readlane_u32(r1, bl));
             }
         }
     }
