@@ -423,7 +423,13 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                    uint32_t pu1) __attribute__((always_inline)) {
         // claimed dim-0 offsets of this lane's cell in this row
         bool useL = false, useC = false, useR = false;
-        if (valid) {
+        if (valid && given && !originRow && a.form != SJ_FORM_BUNICOMP) {
+            // a row the lookup kept off the origin row: under FULL, LID and
+            // Unicomp every lane claims every cell of it
+            useL = x >= 1;
+            useC = true;
+            useR = x + 1 < nC0;
+        } else if (valid) {
             int oo[K];
 #pragma unroll
             for (int j = 0; j < K; j++) oo[j] = o[j];
