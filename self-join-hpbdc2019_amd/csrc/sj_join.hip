@@ -423,7 +423,10 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                    uint32_t pu1) __attribute__((always_inline)) {
         // claimed dim-0 offsets of this lane's cell in this row
         bool useL = false, useC = false, useR = false;
-        if (valid && given && !originRow && a.form != SJ_FORM_BUNICOMP) {
+        // every lane claims every cell of the row: a lane's span has no
+        // unclaimed gap (uniform)
+        const bool allClaimed = given && !originRow && a.form != SJ_FORM_BUNICOMP;
+        if (valid && allClaimed) {
             // a row the lookup kept off the origin row: under FULL, LID and
             // Unicomp every lane claims every cell of it
             useL = x >= 1;
@@ -559,10 +562,14 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     uint64_t okm = 0ull;
                     uint32_t r0 = 0u;
                     if (base < len) {
-                        if (useL) okm |= range_bits(pstart, e0, e1);
-                        if (useC) okm |= range_bits(pstart, e1, e2);
-                        if (useR) okm |= range_bits(pstart, e2, e3);
-                        okm &= range_bits(pstart, pstart, clo + len);
+                        if (allClaimed) {
+                            okm = range_bits(pstart, pstart, clo + len);
+                        } else {
+                            if (useL) okm |= range_bits(pstart, e0, e1);
+                            if (useC) okm |= range_bits(pstart, e1, e2);
+                            if (useR) okm |= range_bits(pstart, e2, e3);
+                            okm &= range_bits(pstart, pstart, clo + len);
+                        }
                         r0 = pstart - c0;
                     }
                     // uniform trip count, US candidates per step, one bit
@@ -621,8 +628,13 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                     // the rows written) and own hits of the piece
                     uint64_t mirBits = 0ull;
                     if (anyMirror) {
-                        uint64_t elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
-                        if (mirC) elig |= range_bits(pstart, e1, e2);
+                        // every hit is eligible when the lane claimed all cells
+                        // off the origin row
+                        uint64_t elig = ~0ull;
+                        if (!allClaimed) {
+                            elig = range_bits(pstart, e0, e1) | range_bits(pstart, e2, e3);
+                            if (mirC) elig |= range_bits(pstart, e1, e2);
+                        }
                         mirBits = mask & elig & range_bits(pstart, a.wBegin, a.wEnd);
                     }
                     const uint32_t myCnt = (uint32_t)__popcll(mask);
