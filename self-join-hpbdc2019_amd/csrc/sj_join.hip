@@ -226,6 +226,18 @@ struct TileLds {
 // most OWN_STAGE), then stored 64 consecutive entries at a time.
 constexpr uint32_t OWN_STAGE = 384;
 
+// LDS of a count pass: the staged coordinates and the mirrored-hit
+// counters (the candidate IDs and reservation slots belong to the fills).
+template <int D, int PREC>
+struct TileLdsCount {
+    using C = TileCfg<D, PREC>;
+    double x64[PREC == 1 ? 1 : C::CH + 64][C::DD];
+    float x32[PREC == 1 ? C::CH + 64 : 1][C::DD];
+    int32_t id[1];
+    uint32_t cnt[C::CH];
+    unsigned long long base[1];
+};
+
 // LDS of the fill that replays a hit record: candidate IDs and mirrored-hit
 // counters, chunked like the count pass of precision PREC, and the own-hit
 // stage: entry i is neighbour ID sid[i] for position wc[l] + t of lane l's
@@ -1027,6 +1039,14 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(3))) void k_
     join_body<K, D, PASS, PREC, false>(a, lds);
 }
 
+// 5 <= k <= 7, FP64 count: four waves per SIMD, which its registers (at
+// most 128) and the count-only LDS allow
+template <int K, int D>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_join_c4(JoinArgs a) {
+    __shared__ TileLdsCount<D, 0> lds[WPB];
+    join_body<K, D, PASS_COUNT, 0, false>(a, lds);
+}
+
 // Fill that replays the hit record of a count pass of precision PREC
 template <int K, int D, int PREC>
 __global__ __launch_bounds__(JB) void k_join_rp(JoinArgs a) {
@@ -1462,7 +1482,7 @@ static JoinKernel pick_pass(int pass, int prec) {
         if (pass == PASS_COUNT) return prec ? k_join_o4<K, D, PASS_COUNT, 1> : k_join_o4<K, D, PASS_COUNT, 0>;
         return prec ? k_join_o4<K, D, PASS_FILL, 1> : k_join_o4<K, D, PASS_FILL, 0>;
     } else if constexpr (K >= 5 && K <= 7 && D > 0) {
-        if (pass == PASS_COUNT) return prec ? k_join_o3<K, D, PASS_COUNT, 1> : k_join_o3<K, D, PASS_COUNT, 0>;
+        if (pass == PASS_COUNT) return prec ? k_join_o3<K, D, PASS_COUNT, 1> : k_join_c4<K, D>;
         return prec ? k_join_o3<K, D, PASS_FILL, 1> : k_join_o3<K, D, PASS_FILL, 0>;
     } else {
         if (pass == PASS_COUNT) return prec ? k_join<K, D, PASS_COUNT, 1> : k_join<K, D, PASS_COUNT, 0>;
