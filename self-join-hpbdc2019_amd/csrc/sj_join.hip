@@ -1085,6 +1085,18 @@ struct LdsMax<0> {
     static constexpr int v = cmax<(int)sizeof(WideLds), (int)sizeof(MfmaLds<64>)>::v;
 };
 constexpr int MANY_LDS = (LdsMax<DEV_MAXK>::v + 15) / 16 * 16;
+// FP32 instances none of which uses the matrix cores: the FP32 LDS tile
+// walks only
+template <int D>
+struct LdsMax32 {
+    static constexpr int v =
+        cmax<cmax<(int)sizeof(TileLds<D, 1>), (int)sizeof(TileLdsIds<D, 1>)>::v, LdsMax32<D - 1>::v>::v;
+};
+template <>
+struct LdsMax32<0> {
+    static constexpr int v = 0;
+};
+constexpr int MANY_LDS32 = (LdsMax32<DEV_MAXK>::v + 15) / 16 * 16;
 
 template <int K, int PASS, int PREC, bool RP>
 __device__ __forceinline__ void many_tile(const JoinArgs &a, uint32_t tile, unsigned char *raw, LaneStats &st,
@@ -1117,16 +1129,15 @@ __device__ __forceinline__ void many_mfma(const JoinArgs &a, uint32_t tile, unsi
 
 // RP (fill): the instances' LDS tile and matrix-core walks replay the hit
 // record of the count kernel; a record marked unusable makes every item an
-// error instead.
+// error instead.  MF = 2: FP32 instances without matrix-core walks, whose
+// smaller LDS and register needs allow three waves per SIMD.
 template <int PASS, int MF, bool RP>
-__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_many(const JoinArgs *__restrict__ args,
-                                                  const uint32_t *__restrict__ workInst,
-                                                  const uint32_t *__restrict__ workTile, uint32_t W,
-                                                  uint32_t *qctr, uint32_t chunk, int useQueue) {
-    __shared__ __align__(16) unsigned char lds[WPB][MANY_LDS];
+__device__ __forceinline__ void many_body(const JoinArgs *__restrict__ args, const uint32_t *__restrict__ workInst,
+                                          const uint32_t *__restrict__ workTile, uint32_t W, uint32_t *qctr,
+                                          uint32_t chunk, int useQueue, unsigned char *rawBase, int ldsBytes) {
     const unsigned wave = threadIdx.x >> 6;
     const unsigned lane = __lane_id();
-    unsigned char *raw = lds[wave];
+    unsigned char *raw = rawBase + (size_t)wave * ldsBytes;
     LaneStats st;
     TileRec R;  // the wave's block of record words, across its items
     uint32_t cur = 0xFFFFFFFFu;
@@ -1144,7 +1155,7 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
             if (lane == 0) st.err++;
             return;
         }
-        if (MF && a.mDP > 0) {
+        if (MF == 1 && a.mDP > 0) {
             switch (a.mDP) {
             case 4: many_mfma<4, PASS, RP>(a, tile, raw, cleared, st, R); break;
             case 8: many_mfma<8, PASS, RP>(a, tile, raw, cleared, st, R); break;
@@ -1156,14 +1167,14 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
         cleared = 0;
         switch (a.k) {
-        case 1: many_tile<1, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 2: many_tile<2, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 3: many_tile<3, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 4: many_tile<4, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 5: many_tile<5, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 6: many_tile<6, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 7: many_tile<7, PASS, MF, RP>(a, tile, raw, st, R); break;
-        case 8: many_tile<8, PASS, MF, RP>(a, tile, raw, st, R); break;
+        case 1: many_tile<1, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 2: many_tile<2, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 3: many_tile<3, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 4: many_tile<4, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 5: many_tile<5, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 6: many_tile<6, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 7: many_tile<7, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
+        case 8: many_tile<8, PASS, (MF ? 1 : 0), RP>(a, tile, raw, st, R); break;
         }
     };
     if (useQueue) {
@@ -1192,6 +1203,24 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_
             atomicAdd(&args[cur].recCtr[REC_SLICE0 + sl * REC_STRIDE + 1], (unsigned long long)(R.want + REC_BLK));
         }
     }
+}
+
+template <int PASS, int MF, bool RP>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(2))) void k_join_many(const JoinArgs *__restrict__ args,
+                                                  const uint32_t *__restrict__ workInst,
+                                                  const uint32_t *__restrict__ workTile, uint32_t W,
+                                                  uint32_t *qctr, uint32_t chunk, int useQueue) {
+    __shared__ __align__(16) unsigned char lds[WPB][MANY_LDS];
+    many_body<PASS, MF, RP>(args, workInst, workTile, W, qctr, chunk, useQueue, &lds[0][0], MANY_LDS);
+}
+
+template <int PASS, bool RP>
+__global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(3))) void k_join_many32(const JoinArgs *__restrict__ args,
+                                                    const uint32_t *__restrict__ workInst,
+                                                    const uint32_t *__restrict__ workTile, uint32_t W,
+                                                    uint32_t *qctr, uint32_t chunk, int useQueue) {
+    __shared__ __align__(16) unsigned char lds[WPB][MANY_LDS32];
+    many_body<PASS, 2, RP>(args, workInst, workTile, W, qctr, chunk, useQueue, &lds[0][0], MANY_LDS32);
 }
 
 // Tiles: runs of at most 64 consecutive points of one cell row within
@@ -1815,7 +1844,10 @@ sj_status launch_join_many(const JoinArgs *d_args, const uint32_t *workInst, con
     using ManyKernel = void (*)(const JoinArgs *, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, uint32_t,
                                 int);
     ManyKernel kfn;
-    if (pass == PASS_COUNT) kfn = mfma ? k_join_many<PASS_COUNT, 1, false> : k_join_many<PASS_COUNT, 0, false>;
+    if (mfma == 2)  // FP32, no matrix-core instance
+        kfn = pass == PASS_COUNT ? k_join_many32<PASS_COUNT, false>
+                                 : (replay ? k_join_many32<PASS_FILL, true> : k_join_many32<PASS_FILL, false>);
+    else if (pass == PASS_COUNT) kfn = mfma ? k_join_many<PASS_COUNT, 1, false> : k_join_many<PASS_COUNT, 0, false>;
     else if (replay) kfn = mfma ? k_join_many<PASS_FILL, 1, true> : k_join_many<PASS_FILL, 0, true>;
     else kfn = mfma ? k_join_many<PASS_FILL, 1, false> : k_join_many<PASS_FILL, 0, false>;
     hipLaunchKernelGGL(kfn, dim3(blocks), dim3(JB), 0, s, d_args, workInst, workTile, W, qctr, chunk, useQueue);

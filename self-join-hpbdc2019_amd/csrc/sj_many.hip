@@ -101,7 +101,7 @@ struct sj_many_s {
     std::vector<uint64_t> hIdsBase;
     std::vector<unsigned long long> hStat;
     double ms_plan = 0.0, ms_grid = 0.0, ms_run = 0.0;
-    int mfma = 0;  // 1: every instance runs the matrix-core FP32 walk
+    int mfma = 0;  // FP32 instances: 1 with, 2 without matrix-core walks
     // hit record shared by the instances: the count kernel records, the fill
     // kernel replays (the wide walk's instances evaluate again)
     bool rec = false;
@@ -337,6 +337,12 @@ sj_status plan(sj_many_s *m, const sj_instance *in, int count) {
     m->N = N;
     m->W = (uint32_t)W;
     m->mfma = P.precision == SJ_PREC_FP32_RECHECK ? 1 : 0;
+    if (m->mfma) {
+        // no instance on the matrix cores: the FP32 LDS tile walks only
+        bool anyMfma = false;
+        for (int i = 0; i < count; i++) anyMfma = anyMfma || m->args[i].mDP > 0;
+        if (!anyMfma) m->mfma = 2;
+    }
     hInst.reserve(W);
     hTile.reserve(W);
     for (int i = 0; i < count; i++)
