@@ -77,7 +77,7 @@ __device__ __forceinline__ bool decide(const JoinArgs &a, const double *p, const
 
 // ------------------------------------------------------- per-lane fallback
 // One query point, neighbour cells one at a time straight from memory.
-template <int K, int D, int PASS, int PREC>
+template <int K, int D, int PASS, int PREC, bool DN = false>
 __device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s, LaneStats &st) {
     constexpr int DR = D > 0 ? D : 1;
     const int d = D > 0 ? D : a.d;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void process_point(const JoinArgs &a, uint32_t s, Lan
         if (!zero && inb) {
             const uint64_t linB = (uint64_t)((int64_t)linA + dl);
             if (claims<K>(a.form, c, o, linA, linB)) {
-                const int64_t b = find_cell(a, linB);
+                const int64_t b = find_cell_t<DN>(a, linB);
                 if (b >= 0) visit((uint32_t)b, anyMirror);
             }
         }
@@ -302,7 +302,7 @@ __device__ __forceinline__ uint32_t tile_piece_bound(const JoinArgs &a, const ui
 // One tile of <= 64 points of one cell row against its 3^(k-1) neighbour
 // rows.  RP (fill only): take the hit masks from the count pass's record
 // instead of evaluating distances; L is then a TileLdsIds.
-template <int K, int D, int PASS, int PREC, bool RP, class LDS>
+template <int K, int D, int PASS, int PREC, bool RP, class LDS, bool DN = false>
 __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, LDS &L, LaneStats &st,
                                              TileRec &R) {
     using Cfg = TileCfg<D, PREC>;
@@ -333,7 +333,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
         // the tile crosses a cell row: per-lane walk (FP64: the per-lane
         // FP32 walk needs the global FP32 copy, which tiles do not make)
         // a record cannot describe it: the count marks the record unusable
-        if (valid) process_point<K, D, PASS, 0>(a, s, st);
+        if (valid) process_point<K, D, PASS, 0, DN>(a, s, st);
         if (PASS == PASS_COUNT && lane == (unsigned)firstL) {
             st.fallback++;
             if (a.recDir) atomicAdd(&a.recCtr[REC_BAD], 1ull);
@@ -456,8 +456,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
             useC = originRow || claims<K>(a.form, c, oo, linA, rb + x);
         }
         if (__ballot(useL || useC || useR)) {
-            const uint32_t u0 = given ? pu0 : pos_lb(a, rb + ux0);
-            const uint32_t u1 = given ? pu1 : pos_lb(a, rb + ux1 + 1);
+            const uint32_t u0 = given ? pu0 : pos_lb_t<DN>(a, rb + ux0);
+            const uint32_t u1 = given ? pu1 : pos_lb_t<DN>(a, rb + ux1 + 1);
             uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
             // given: the first chunk is staged while the window bounds load
             // (both requests in flight together); the lookups run on every
@@ -466,8 +466,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
             if (given) {
                 const uint32_t xl = x >= 1 ? x - 1 : x;
                 const uint32_t xr = x + 1 < nC0 ? x + 2 : x + 1;
-                const uint32_t f0 = pos_lb(a, rb + xl), f1 = pos_lb(a, rb + x);
-                const uint32_t f2 = pos_lb(a, rb + x + 1), f3 = pos_lb(a, rb + xr);
+                const uint32_t f0 = pos_lb_t<DN>(a, rb + xl), f1 = pos_lb_t<DN>(a, rb + x);
+                const uint32_t f2 = pos_lb_t<DN>(a, rb + x + 1), f3 = pos_lb_t<DN>(a, rb + xr);
                 if constexpr (!RP) {
                     const uint32_t m0 = min((uint32_t)CH, u1 - u0);
                     if (!(PASS == PASS_FILL && !anyOwnW && (u0 + m0 <= a.wBegin || u0 >= a.wEnd))) {
@@ -480,10 +480,10 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                 e2 = valid ? f2 : 0u;
                 e3 = valid ? f3 : 0u;
             } else if (valid) {
-                e1 = pos_lb(a, rb + x);
-                e2 = pos_lb(a, rb + x + 1);
-                e0 = x >= 1 ? pos_lb(a, rb + x - 1) : e1;
-                e3 = x + 1 < nC0 ? pos_lb(a, rb + x + 2) : e2;
+                e1 = pos_lb_t<DN>(a, rb + x);
+                e2 = pos_lb_t<DN>(a, rb + x + 1);
+                e0 = x >= 1 ? pos_lb_t<DN>(a, rb + x - 1) : e1;
+                e3 = x + 1 < nC0 ? pos_lb_t<DN>(a, rb + x + 2) : e2;
             }
             // this lane's span: from the first to the last claimed cell;
             // unclaimed cells inside it are masked per candidate
@@ -928,8 +928,8 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
                 if (a.form == SJ_FORM_LID) may = org || dl > 0;
                 else if (a.form == SJ_FORM_UNICOMP && !org) may = claims<K>(a.form, c, od, linA, rbl);
                 if (may) {
-                    r0 = pos_lb(a, rbl + ux0);
-                    r1 = pos_lb(a, rbl + ux1 + 1);
+                    r0 = pos_lb_t<DN>(a, rbl + ux0);
+                    r1 = pos_lb_t<DN>(a, rbl + ux1 + 1);
                     act = r1 > r0;
                 }
             }
@@ -967,7 +967,7 @@ __device__ __forceinline__ void process_tile(const JoinArgs &a, uint32_t tile, L
     st.err += err;
 }
 
-template <int K, int D, int PASS, int PREC, bool RP, class LDS>
+template <int K, int D, int PASS, int PREC, bool RP, class LDS, bool DN = false>
 __device__ __forceinline__ void join_body(const JoinArgs &a, LDS *lds) {
     if (PASS == PASS_FILL && !fill_gate(a)) return;
     const unsigned wave = threadIdx.x >> 6;
@@ -978,12 +978,12 @@ __device__ __forceinline__ void join_body(const JoinArgs &a, LDS *lds) {
     TileRec R;
     auto run = [&](uint32_t tile) __attribute__((always_inline)) {
         if constexpr (RP) {
-            process_tile<K, D, PASS, PREC, true>(a, tile, L, st, R);
+            process_tile<K, D, PASS, PREC, true, LDS, DN>(a, tile, L, st, R);
         } else if (D > 0 && a.useTiles) {
-            process_tile<K, D, PASS, PREC, false>(a, tile, L, st, R);
+            process_tile<K, D, PASS, PREC, false, LDS, DN>(a, tile, L, st, R);
         } else {
             const uint64_t s = (uint64_t)a.tileStart[tile] + lane;
-            if (s < a.tileStart[tile + 1]) process_point<K, D, PASS, PREC>(a, (uint32_t)s, st);
+            if (s < a.tileStart[tile + 1]) process_point<K, D, PASS, PREC, DN>(a, (uint32_t)s, st);
         }
     };
     if (a.tileOrder) {
@@ -1024,10 +1024,10 @@ __global__ __launch_bounds__(JB) void k_join(JoinArgs a) {
 
 // k <= 2: four waves per SIMD (at most 128 VGPRs); measured faster at
 // 2-D than the wider allocation the compiler picks by itself
-template <int K, int D, int PASS, int PREC>
+template <int K, int D, int PASS, int PREC, bool DN = false>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_join_o4(JoinArgs a) {
     __shared__ TileLds<D, PREC> lds[WPB];
-    join_body<K, D, PASS, PREC, false>(a, lds);
+    join_body<K, D, PASS, PREC, false, TileLds<D, PREC>, DN>(a, lds);
 }
 
 // 5 <= k <= 7: three waves per SIMD (at most 168 VGPRs), which the LDS
@@ -1056,10 +1056,10 @@ __global__ __launch_bounds__(JB) void k_join_rp(JoinArgs a) {
 
 // k <= 2: five waves per SIMD (at most 96 VGPRs); measured faster at 2-D,
 // slower at 6-D
-template <int K, int D, int PREC>
+template <int K, int D, int PREC, bool DN = false>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(5))) void k_join_rp_o5(JoinArgs a) {
     __shared__ TileLdsIds<D, PREC> lds[WPB];
-    join_body<K, D, PASS_FILL, PREC, true>(a, lds);
+    join_body<K, D, PASS_FILL, PREC, true, TileLdsIds<D, PREC>, DN>(a, lds);
 }
 
 // ------------------------------------------------------------ many joins
@@ -1790,6 +1790,13 @@ static JoinKernel pick_replay_k(int prec) {
     else return prec ? k_join_rp<K, K, 1> : k_join_rp<K, K, 0>;
 }
 
+// k = d = 2 on a dense grid: kernels without the binary-search lookups
+static JoinKernel pick_dense2(int pass, int prec, bool replay) {
+    if (replay) return prec ? k_join_rp_o5<2, 2, 1, true> : k_join_rp_o5<2, 2, 0, true>;
+    if (pass == PASS_COUNT) return prec ? k_join_o4<2, 2, PASS_COUNT, 1, true> : k_join_o4<2, 2, PASS_COUNT, 0, true>;
+    return prec ? k_join_o4<2, 2, PASS_FILL, 1, true> : k_join_o4<2, 2, PASS_FILL, 0, true>;
+}
+
 static JoinKernel pick_replay(int k, int prec) {
     switch (k) {
     case 1: return pick_replay_k<1>(prec);
@@ -1821,6 +1828,7 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
     if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
         return launch_join_wide(a, pass, g->ctx->cu_count, s);
     JoinKernel kfn = replay ? pick_replay(g->k, prec) : pick_kernel(g->k, g->d, pass, prec);
+    if (a.posLB && g->k == 2 && g->d == 2) kfn = pick_dense2(pass, prec, replay);
     if (!kfn) {
         set_error("no traversal kernel for k=%d", g->k);
         return SJ_ERR_UNSUPPORTED;

@@ -245,6 +245,25 @@ __device__ __forceinline__ bool fill_gate(const JoinArgs &a) {
 }
 
 // first sorted position whose cell has linear ID >= lin
+__device__ __forceinline__ uint32_t pos_lb(const JoinArgs &a, uint64_t lin);
+
+// DN: the grid has the dense table (kernels built for it alone carry no
+// binary search)
+template <bool DN>
+__device__ __forceinline__ uint32_t pos_lb_t(const JoinArgs &a, uint64_t lin) {
+    if constexpr (DN) return a.posLB[lin];
+    else return pos_lb(a, lin);
+}
+template <bool DN>
+__device__ __forceinline__ int64_t find_cell_t(const JoinArgs &a, uint64_t lin) {
+    if constexpr (DN) {
+        const uint32_t p0 = a.posLB[lin];
+        return p0 < a.posLB[lin + 1] ? (int64_t)a.cellOf[p0] : -1;
+    } else {
+        return find_cell(a, lin);
+    }
+}
+
 __device__ __forceinline__ uint32_t pos_lb(const JoinArgs &a, uint64_t lin) {
     if (a.posLB) return a.posLB[lin];
     uint64_t lo = 0, hi = a.nne;
