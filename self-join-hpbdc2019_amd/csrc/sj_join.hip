@@ -1033,25 +1033,25 @@ __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // 5 <= k <= 7: three waves per SIMD (at most 168 VGPRs), which the LDS
 // of these tiles still allows; measured faster at 6-D than the two waves
 // the compiler's own allocation leaves
-template <int K, int D, int PASS, int PREC>
+template <int K, int D, int PASS, int PREC, bool DN = false>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(3))) void k_join_o3(JoinArgs a) {
     __shared__ TileLds<D, PREC> lds[WPB];
-    join_body<K, D, PASS, PREC, false>(a, lds);
+    join_body<K, D, PASS, PREC, false, TileLds<D, PREC>, DN>(a, lds);
 }
 
 // 5 <= k <= 7, FP64 count: four waves per SIMD, which its registers (at
 // most 128) and the count-only LDS allow
-template <int K, int D>
+template <int K, int D, bool DN = false>
 __global__ __launch_bounds__(JB) __attribute__((amdgpu_waves_per_eu(4))) void k_join_c4(JoinArgs a) {
     __shared__ TileLdsCount<D, 0> lds[WPB];
-    join_body<K, D, PASS_COUNT, 0, false>(a, lds);
+    join_body<K, D, PASS_COUNT, 0, false, TileLdsCount<D, 0>, DN>(a, lds);
 }
 
 // Fill that replays the hit record of a count pass of precision PREC
-template <int K, int D, int PREC>
+template <int K, int D, int PREC, bool DN = false>
 __global__ __launch_bounds__(JB) void k_join_rp(JoinArgs a) {
     __shared__ TileLdsIds<D, PREC> lds[WPB];
-    join_body<K, D, PASS_FILL, PREC, true>(a, lds);
+    join_body<K, D, PASS_FILL, PREC, true, TileLdsIds<D, PREC>, DN>(a, lds);
 }
 
 // k <= 2: five waves per SIMD (at most 96 VGPRs); measured faster at 2-D,
@@ -1790,11 +1790,18 @@ static JoinKernel pick_replay_k(int prec) {
     else return prec ? k_join_rp<K, K, 1> : k_join_rp<K, K, 0>;
 }
 
-// k = d = 2 on a dense grid: kernels without the binary-search lookups
-static JoinKernel pick_dense2(int pass, int prec, bool replay) {
-    if (replay) return prec ? k_join_rp_o5<2, 2, 1, true> : k_join_rp_o5<2, 2, 0, true>;
-    if (pass == PASS_COUNT) return prec ? k_join_o4<2, 2, PASS_COUNT, 1, true> : k_join_o4<2, 2, PASS_COUNT, 0, true>;
-    return prec ? k_join_o4<2, 2, PASS_FILL, 1, true> : k_join_o4<2, 2, PASS_FILL, 0, true>;
+// k = d = 2 or 6 on a dense grid: kernels without the binary-search
+// lookups (the 2-D and 6-D shapes; other k use the general kernels)
+static JoinKernel pick_dense(int k, int pass, int prec, bool replay) {
+    if (k == 2) {
+        if (replay) return prec ? k_join_rp_o5<2, 2, 1, true> : k_join_rp_o5<2, 2, 0, true>;
+        if (pass == PASS_COUNT)
+            return prec ? k_join_o4<2, 2, PASS_COUNT, 1, true> : k_join_o4<2, 2, PASS_COUNT, 0, true>;
+        return prec ? k_join_o4<2, 2, PASS_FILL, 1, true> : k_join_o4<2, 2, PASS_FILL, 0, true>;
+    }
+    if (replay) return prec ? k_join_rp<6, 6, 1, true> : k_join_rp<6, 6, 0, true>;
+    if (pass == PASS_COUNT) return prec ? k_join_o3<6, 6, PASS_COUNT, 1, true> : k_join_c4<6, 6, true>;
+    return prec ? k_join_o3<6, 6, PASS_FILL, 1, true> : k_join_o3<6, 6, PASS_FILL, 0, true>;
 }
 
 static JoinKernel pick_replay(int k, int prec) {
@@ -1828,7 +1835,7 @@ static sj_status launch_join(sj_grid_s *g, const JoinArgs &a, int pass, int prec
     if (prec == SJ_PREC_FP64 && a.useTiles && wide_supported(g->d, g->k))
         return launch_join_wide(a, pass, g->ctx->cu_count, s);
     JoinKernel kfn = replay ? pick_replay(g->k, prec) : pick_kernel(g->k, g->d, pass, prec);
-    if (a.posLB && g->k == 2 && g->d == 2) kfn = pick_dense2(pass, prec, replay);
+    if (a.posLB && g->k == g->d && (g->k == 2 || g->k == 6)) kfn = pick_dense(g->k, pass, prec, replay);
     if (!kfn) {
         set_error("no traversal kernel for k=%d", g->k);
         return SJ_ERR_UNSUPPORTED;
