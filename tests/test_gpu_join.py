@@ -39,6 +39,9 @@ CASES = {
     "uniform4d_40k": lambda: (D.uniform(40_000, 4, 12), 4, 0.07),
     "clusters64d_k6_8k": lambda: (D.clusters(8_000, 64, 4), 6, 4.5),
     "uniform8d_k5_20k": lambda: (D.uniform(20_000, 8, 13), 5, 0.35),
+    # d = k = 5 and 7: the tile walks built for k = 5..7 (four-wave FP64 count)
+    "uniform5d_20k": lambda: (D.uniform(20_000, 5, 14), 5, 0.18),
+    "uniform7d_20k": lambda: (D.uniform(20_000, 7, 15), 7, 0.30),
 }
 
 _cache = {}
@@ -90,7 +93,8 @@ def test_join_fp64_matches_oracle(ctx, name, form, walk):
     g.free()
 
 
-@pytest.mark.parametrize("name", ["uniform2d_100k", "exp2d_200k", "uniform6d_50k", "clusters64d_k6_8k"])
+@pytest.mark.parametrize("name", ["uniform2d_100k", "exp2d_200k", "uniform6d_50k", "clusters64d_k6_8k",
+                                  "uniform5d_20k", "uniform7d_20k"])
 @pytest.mark.parametrize("form", ["bunicomp", "lid"])
 def test_join_fp32_recheck_exact(ctx, name, form):
     import sjamd
@@ -197,6 +201,32 @@ def test_grid_key_width_from_last_build(ctx):
         _assert_same(r.csr(), og.join(O.LID, sort_rows=True))
         r.free()
         g.free()
+
+
+@pytest.mark.parametrize("case", ["clusters2d_sparse", "clusters6d_sparse"])
+def test_sparse_grids_general_kernels(ctx, case):
+    """Grids too sparse for the dense lookup table (totalCells above 2^26
+    and above 16 n): the 2-D and 6-D joins then run the kernels with the
+    binary-search lookups rather than the dense-grid variants, and must
+    equal the oracle in every form and both precisions."""
+    import sjamd
+
+    if case == "clusters2d_sparse":
+        pts, k, eps = D.clusters(20_000, 2, 16, n_clusters=20, box=1e5, sigma=2.0), 2, 1.0
+    else:
+        pts, k, eps = D.clusters(20_000, 6, 17, n_clusters=20, box=100.0, sigma=0.3), 6, 0.5
+    ref = O.join(pts, k, eps, O.LID, sort_rows=True)
+    g = ctx.grid_build(pts, k, eps)
+    n = pts.shape[0]
+    tc = g.dims()[3]
+    assert tc > (1 << 26) and tc > 16 * n and not g.info.dense_lookup
+    for form in ("full", "unicomp", "bunicomp", "lid"):
+        for prec in (sjamd.FP64, sjamd.FP32_RECHECK):
+            _say(f"[sparse] {case} form={form} prec={prec}")
+            r = g.join(form=form, precision=prec)
+            _assert_same(r.csr(), ref)
+            r.free()
+    g.free()
 
 
 def test_edge_single_point(ctx):
