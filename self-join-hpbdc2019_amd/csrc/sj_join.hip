@@ -2282,8 +2282,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             ar.recGate = recCtr.p + REC_BAD;
             ar.recGateMode = 1;  // replay: the record is usable
             SJ_TRY(launch_join(g, ar, PASS_FILL, prec, true));
-            ar.recGateMode = 2;  // evaluate: the record is not
-            if (a.tileOrder) SJ_TRY(fill_u32(s, qctr.p, QSLICES * QSTRIDE, 0u));
+            // evaluate: the record is not.  Exactly one of the two runs, so
+            // the queue counters the cursor kernel zeroed are still zero
+            // when this one does
+            ar.recGateMode = 2;
             SJ_TRY(launch_join(g, ar, PASS_FILL, prec, false));
         }
         return SJ_OK;
