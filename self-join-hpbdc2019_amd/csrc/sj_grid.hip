@@ -20,11 +20,12 @@ namespace sj {
 constexpr int GB = 256;
 
 // Per-block min/max of the first k coordinates and a check that all d are
-// finite.  DT = d when d <= 8 (the row is loaded in full, four rows in
-// flight per thread), DT = 0 for any d.
+// finite, into partial[block * (2k + 1) ..]: k minima, k maxima, then 1.0
+// when a coordinate was not finite.  DT = d when d <= 8 (the row is loaded
+// in full, four rows in flight per thread), DT = 0 for any d.
 template <int DT>
 __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, int d, int k,
-                                               double *partial, uint32_t *flags) {
+                                               double *partial) {
     constexpr int KM = DT > 0 ? (DT < SJ_MAXK ? DT : SJ_MAXK) : SJ_MAXK;
     constexpr int U = DT > 0 ? 4 : 1;
     double mn[KM], mx[KM];
@@ -68,7 +69,10 @@ __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, in
             }
         }
     }
-    if (bad) atomicOr(flags, (uint32_t)GP_NONFINITE);
+    __shared__ uint32_t sbad;
+    if (threadIdx.x == 0) sbad = 0u;
+    __syncthreads();
+    if (bad) sbad = 1u;
     __shared__ double smn[GB / 64][SJ_MAXK], smx[GB / 64][SJ_MAXK];
     const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
 #pragma unroll
@@ -92,12 +96,13 @@ __global__ __launch_bounds__(GB) void k_minmax(const double *pts, uint64_t n, in
             a = fmin(a, smn[w][j]);
             b = fmax(b, smx[w][j]);
         }
-        partial[(uint64_t)blockIdx.x * 2 * k + j] = a;
-        partial[(uint64_t)blockIdx.x * 2 * k + k + j] = b;
+        partial[(uint64_t)blockIdx.x * (2 * k + 1) + j] = a;
+        partial[(uint64_t)blockIdx.x * (2 * k + 1) + k + j] = b;
     }
+    if (threadIdx.x == 0) partial[(uint64_t)blockIdx.x * (2 * k + 1) + 2 * k] = sbad ? 1.0 : 0.0;
 }
 
-using MinMaxKernel = void (*)(const double *, uint64_t, int, int, double *, uint32_t *);
+using MinMaxKernel = void (*)(const double *, uint64_t, int, int, double *);
 static MinMaxKernel pick_minmax(int d) {
     switch (d) {
     case 1: return k_minmax<1>;
@@ -115,16 +120,24 @@ static MinMaxKernel pick_minmax(int d) {
 // main.cu:454-501: min/max, then minArr -= eps, maxArr += eps,
 // nCells = ceil((max-min)/eps), totalCells = prod nCells (dim 0 fastest).
 // One block: the block reduces the per-block partials of each dimension.
+// The error flags start here: flags[0] = GP_NONFINITE when a coordinate was
+// not finite, 0 otherwise (later kernels add their bits).
 __global__ __launch_bounds__(GB) void k_grid_params(const double *partial, int nb, int k, double eps,
-                                                    GridParams *gp, const uint32_t *flags) {
+                                                    GridParams *gp, uint32_t *flags) {
     __shared__ double smn[GB / 64][SJ_MAXK], smx[GB / 64][SJ_MAXK];
     __shared__ double fmn[SJ_MAXK], fmx[SJ_MAXK];
+    __shared__ uint32_t sbad;
     const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
+    const int rs = 2 * k + 1;
+    if (threadIdx.x == 0) sbad = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += GB)
+        if (partial[(uint64_t)i * rs + 2 * k] != 0.0) sbad = 1u;
     for (int j = 0; j < k; j++) {
         double a = INFINITY, b = -INFINITY;
         for (int i = threadIdx.x; i < nb; i += GB) {
-            a = fmin(a, partial[(uint64_t)i * 2 * k + j]);
-            b = fmax(b, partial[(uint64_t)i * 2 * k + k + j]);
+            a = fmin(a, partial[(uint64_t)i * rs + j]);
+            b = fmax(b, partial[(uint64_t)i * rs + k + j]);
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -149,7 +162,8 @@ __global__ __launch_bounds__(GB) void k_grid_params(const double *partial, int n
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    uint32_t f = *flags;
+    uint32_t f = sbad ? (uint32_t)GP_NONFINITE : 0u;
+    *flags = f;  // the parameters' own overflow bits below go to gp->flags
     uint64_t tot = 1, mul = 1;
     for (int j = 0; j < k; j++) {
         const double a = fmn[j] - eps;
@@ -380,11 +394,10 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     const unsigned nbmm = (unsigned)std::min<uint64_t>(1024, blocks_for(n));
     DBuf<double> partial;
     DBuf<uint32_t> flags;
-    SJ_TRY(partial.alloc(pool, (uint64_t)nbmm * 2 * k));
-    SJ_TRY(flags.alloc(pool, 2));  // error flags, then the non-empty cell count
+    SJ_TRY(partial.alloc(pool, (uint64_t)nbmm * (2 * k + 1)));
+    SJ_TRY(flags.alloc(pool, 2));  // error flags (set first by k_grid_params), then the non-empty cell count
     SJ_TRY(g->d_gp.alloc(pool, 1));
-    SJ_TRY(fill_u32(s, flags.p, 1, 0u));
-    hipLaunchKernelGGL(pick_minmax(d), dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p, flags.p);
+    hipLaunchKernelGGL(pick_minmax(d), dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p);
     SJ_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
                        flags.p);

@@ -1397,21 +1397,26 @@ __global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, u
 // |S|, out[1..8] = statistics, out[16] = record unusable, out[18] =
 // directory entries the record needed (the last two when recording).
 // The count's error counter is cleared once read, for the fill's errors.
-__global__ void k_count_scalars(const uint64_t *total, unsigned long long *stat,
-                                const unsigned long long *recCtr, const uint64_t *dirTotal, uint64_t *out) {
-    const unsigned t = threadIdx.x;
-    if (t == 0) out[0] = *total;
-    if (t >= 1 && t <= 8) out[t] = stat[t - 1];
-    if (t == 1 + STAT_ERR) stat[STAT_ERR] = 0ull;
-    if (t == 16) out[16] = recCtr ? recCtr[REC_BAD] : 0ull;
-    if (t == 18) out[18] = dirTotal ? *dirTotal : 0ull;
+// The live statistics sit at out[SC_STAT..] of the same buffer, so one copy
+// of it after the fill also carries the fill's errors and overflow flag.
+constexpr int SC_STAT = 24, SC_WORDS = 32;
+struct CountScalars {
+    const uint64_t *total;
+    unsigned long long *stat;
+    const unsigned long long *recCtr;  // null: not recording
+    const uint64_t *dirTotal;          // null: not recording
+    uint64_t *out;                     // null: nothing to gather
+};
+
+__device__ __forceinline__ void gather_count_scalars(const CountScalars &c, unsigned t) {
+    if (t == 0) c.out[0] = *c.total;
+    if (t >= 1 && t <= 8) c.out[t] = c.stat[t - 1];
+    if (t == 1 + STAT_ERR) c.stat[STAT_ERR] = 0ull;
+    if (t == 16) c.out[16] = c.recCtr ? c.recCtr[REC_BAD] : 0ull;
+    if (t == 18) c.out[18] = c.dirTotal ? *c.dirTotal : 0ull;
 }
 
-// The fill's error count and result-overflow flag after the count's scalars.
-__global__ void k_fill_scalars(const unsigned long long *stat, uint64_t *out) {
-    if (threadIdx.x == 0) out[19] = stat[STAT_ERR];
-    if (threadIdx.x == 1) out[20] = stat[STAT_OVF];
-}
+__global__ void k_count_scalars(CountScalars c) { gather_count_scalars(c, threadIdx.x); }
 
 // The count pass's counters at zero: mirrored-hit counts of the n rows,
 // statistics, and (when not null) the queue slice counters and the
@@ -1430,10 +1435,12 @@ __global__ __launch_bounds__(JB) void k_zero_count_state(uint32_t *mir, uint64_t
 // recBad (when set): only when the hit record is unusable (the fill then
 // evaluates distances again and takes its mirrored slots from the cursors)
 // qctr (when set): the work-queue counters at zero for the fill that follows.
+// sc.out (when set): block 0 also gathers the count pass's scalars.
 __global__ void k_mir_cursors(const uint64_t *offS, const uint32_t *own, uint32_t w0, uint32_t w1,
                               uint64_t outBase, unsigned long long *mirCur, const unsigned long long *recBad,
-                              uint32_t *qctr) {
+                              uint32_t *qctr, CountScalars sc) {
     const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
+    if (sc.out && blockIdx.x == 0) gather_count_scalars(sc, threadIdx.x);
     if (qctr && i < (uint64_t)QSLICES * QSTRIDE) qctr[i] = 0u;
     uint64_t s = (uint64_t)w0 + i;
     if (s >= w1) return;
@@ -2128,13 +2135,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
 
     // count pass
     DBuf<uint32_t> own, mir;
-    DBuf<unsigned long long> stat;
+    DBuf<uint64_t> scal;  // the host's scalars, the live statistics at SC_STAT
     SJ_TRY(own.alloc(pool, n));
     SJ_TRY(mir.alloc(pool, n));
-    SJ_TRY(stat.alloc(pool, 8));
+    SJ_TRY(scal.alloc(pool, SC_WORDS));
+    unsigned long long *const stat = reinterpret_cast<unsigned long long *>(scal.p) + SC_STAT;
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
-    a.stat = stat.p;
+    a.stat = stat;
     // hit record of the count pass for the fill, in space that persists
     // on the context; without the space the fill evaluates distances again
     DBuf<uint64_t> recDir, recMask, tileDir;
@@ -2165,7 +2173,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         }
     }
     const uint64_t nz = std::max<uint64_t>(n, std::max<uint64_t>(REC_NCTR, QSLICES * QSTRIDE));
-    hipLaunchKernelGGL(k_zero_count_state, dim3(nblk(nz)), dim3(JB), 0, s, mir.p, n, stat.p,
+    hipLaunchKernelGGL(k_zero_count_state, dim3(nblk(nz)), dim3(JB), 0, s, mir.p, n, stat,
                        a.tileOrder ? qctr.p : nullptr, recording ? recCtr.p : nullptr);
     SJ_LAUNCH_CHECK();
     tc.start();
@@ -2190,12 +2198,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     // result fitting); the host reads everything once at the end and fills
     // again only if the result outgrew the buffer.
     uint64_t *pin = ctx->pinned;
-    DBuf<uint64_t> scal;
-    SJ_TRY(scal.alloc(pool, 21));
-    hipLaunchKernelGGL(k_count_scalars, dim3(1), dim3(64), 0, s, r->d_offsets.p + n, stat.p,
-                       recording ? recCtr.p : nullptr, recording ? tileDir.p + tBound : nullptr, scal.p);
-    SJ_LAUNCH_CHECK();
+    const CountScalars csc{r->d_offsets.p + n, stat, recording ? recCtr.p : nullptr,
+                           recording ? tileDir.p + tBound : nullptr, scal.p};
     const bool ahead = P->capacity == 0 && !r->extIds && ctx->idsHint > 0;
+    // ahead: the fill's first cursor kernel gathers them
+    if (!ahead) {
+        hipLaunchKernelGGL(k_count_scalars, dim3(1), dim3(64), 0, s, csc);
+        SJ_LAUNCH_CHECK();
+    }
     bool replay = false;
     // the count pass's scalars (sc: k_count_scalars layout) into the result
     auto absorb = [&](const uint64_t *sc) -> sj_status {
@@ -2258,25 +2268,32 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     // places mirrored hits at the recorded reservations, every other fill
     // takes them from the cursors.  known: the record's state is known on the
     // host (replay); otherwise both fills are launched, each gated on it.
-    auto launch_fill = [&](uint64_t cap, bool known) -> sj_status {
+    // gather: the cursor kernel also gathers the count pass's scalars
+    auto launch_fill = [&](uint64_t cap, bool known, bool gather) -> sj_status {
         a.ids = r->d_ids.p;
         a.outBase = 0;
         a.idsCap = cap;
         const bool gate = !known && recording;
         uint32_t *qz = a.tileOrder ? qctr.p : nullptr;  // queue counters, zeroed by the cursor kernel
         const unsigned nbq = (unsigned)std::max<uint64_t>(nblk(n), nblk(QSLICES * QSTRIDE));
+        CountScalars gsc = csc;
+        if (!gather) gsc.out = nullptr;
         if (!gate) {
             if (!(replay && a.mDP == 0)) {
                 hipLaunchKernelGGL(k_mir_cursors, dim3(nbq), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                                   (uint32_t)n, 0ull, mirCur.p, nullptr, qz);
+                                   (uint32_t)n, 0ull, mirCur.p, nullptr, qz, gsc);
                 SJ_LAUNCH_CHECK();
-            } else if (qz) {
-                SJ_TRY(fill_u32(s, qz, QSLICES * QSTRIDE, 0u));
+            } else {
+                if (gather) {
+                    hipLaunchKernelGGL(k_count_scalars, dim3(1), dim3(64), 0, s, csc);
+                    SJ_LAUNCH_CHECK();
+                }
+                if (qz) SJ_TRY(fill_u32(s, qz, QSLICES * QSTRIDE, 0u));
             }
             SJ_TRY(launch_join(g, a, PASS_FILL, prec, replay));
         } else {
             hipLaunchKernelGGL(k_mir_cursors, dim3(nbq), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
-                               (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr, qz);
+                               (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr, qz, gsc);
             SJ_LAUNCH_CHECK();
             JoinArgs ar = a;
             ar.recGate = recCtr.p + REC_BAD;
@@ -2311,30 +2328,28 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         }
         for (int round = 0; round < 2; round++) {
             tf.start();
-            SJ_TRY(launch_fill(known ? 0 : cap, known));
+            SJ_TRY(launch_fill(known ? 0 : cap, known, round == 0 && ahead));
             tf.stop();
             if (P->sort_rows) {
                 tr.start();
                 SJ_TRY(sort_rows_device(ctx, r->d_offsets.p, r->d_ids.p, n, 0, known ? 0 : cap));
                 tr.stop();
             }
-            // the fill's errors and overflow after the count's scalars
-            hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(64), 0, s, stat.p, scal.p);
-            SJ_LAUNCH_CHECK();
-            SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, 21 * 8, hipMemcpyDeviceToHost, s));
+            // the count's scalars and the fill's errors and overflow flag
+            SJ_TRY_HIP(hipMemcpyAsync(pin, scal.p, SC_WORDS * 8, hipMemcpyDeviceToHost, s));
             SJ_TRY_HIP(hipStreamSynchronize(s));
             if (known) break;
             SJ_TRY(absorb(pin));
-            if (!pin[20]) break;
+            if (!pin[SC_STAT + STAT_OVF]) break;
             // the result outgrew the buffer: fill again into one of its size
             known = true;
             r->stats.overflow_retries = 1;
             SJ_TRY(r->d_ids.alloc(pool, r->total));
-            SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat.p) + STAT_ERR, 1, 0ull));
+            SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat) + STAT_ERR, 1, 0ull));
         }
-        if (pin[19]) {
+        if (pin[SC_STAT + STAT_ERR]) {
             set_error("join: fill pass disagreed with the count pass (%llu writes)",
-                      (unsigned long long)pin[19]);
+                      (unsigned long long)pin[SC_STAT + STAT_ERR]);
             return SJ_ERR_STATE;
         }
         r->stats.fill_from_record = replay ? 1 : 0;
@@ -2432,7 +2447,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             const uint32_t t1 = (uint32_t)(std::lower_bound(hTile.p, hTile.p + nT + 1, (uint32_t)q1) - hTile.p);
             const uint64_t base = hOff.p[s0];
             hipLaunchKernelGGL(k_mir_cursors, dim3(nblk(s1 - s0)), dim3(JB), 0, s, r->d_offsets.p, own.p, s0, s1,
-                               (unsigned long long)base, mirCur.p, nullptr, nullptr);
+                               (unsigned long long)base, mirCur.p, nullptr, nullptr, CountScalars{});
             SJ_LAUNCH_CHECK();
             // the buffer's previous download must be done before it is refilled
             if (bi >= 2) SJ_TRY_HIP(hipStreamWaitEvent(s, copied[sl].e, 0));
@@ -2467,7 +2482,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
         }
         SJ_TRY_HIP(hipStreamSynchronize(cs));
         for (auto &t : tfs) ms_fill += t->ms();
-        SJ_TRY_HIP(hipMemcpyAsync(pin, stat.p + STAT_ERR, 8, hipMemcpyDeviceToHost, s));
+        SJ_TRY_HIP(hipMemcpyAsync(pin, stat + STAT_ERR, 8, hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipStreamSynchronize(s));
         if (pin[0]) {
             set_error("join: fill pass disagreed with the count pass (%llu writes)",

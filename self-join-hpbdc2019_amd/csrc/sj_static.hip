@@ -43,7 +43,7 @@ uint64_t scan_state(uint64_t n) { return n / 8 + (1u << 20); }  // look-back sca
 uint64_t radix_hist(uint64_t n) { return n / 2 + 4096; }        // histograms of one sort
 
 uint64_t grid_need(uint64_t n, int d, int k) {
-    uint64_t b = 1024 * 2 * 12 * 8 + 1024;             // min/max partials, flags, grid params
+    uint64_t b = 1024 * (2 * 12 + 1) * 8 + 1024;       // min/max partials, flags, grid params
     b += 4 * n * 8 + radix_hist(n);                     // keys, alternate keys (64-bit), values
     b += 2 * n * 4 + 4 + scan_state(n);                 // head flags, their scan
     b += n * 4 + (n + 1) * 4 + n * 8;                   // cellOf, cellStart, cellLinear
