@@ -261,11 +261,18 @@ __global__ __launch_bounds__(GB) void k_cell_coords(const double *spts, const ui
 // posLB[L], L in [0, tc]: the first sorted position whose cell has linear
 // ID >= L, i.e. cellStart of the first non-empty cell at or after L (n past
 // the last).  A cell L is non-empty iff posLB[L] < posLB[L + 1].  Thread t
-// fills the run [t * POSLB_RUN, (t + 1) * POSLB_RUN).  The wave first finds
-// the first cell at or after its lowest ID with a 64-way search (each round
-// the 64 lanes probe evenly spaced cells); a lane's own first cell then lies
-// within (its ID - the wave's ID) cells of it, found by a short binary
-// search; a forward walk fills the run, stored 16 bytes at a time.
+// fills the run [t * POSLB_RUN, (t + 1) * POSLB_RUN); its wave covers 64
+// runs from Lw.  The loads are kept off long dependent chains:
+//   - the wave finds lo, its first cell at or after Lw, with a 64-way
+//     search (each round the 64 lanes probe evenly spaced cells);
+//   - the lanes load the 64 cells from lo, one each, and a lane counts
+//     those below its own first ID with a 6-step search over the lanes'
+//     registers; a lane with all 64 below (a dense stretch) searches the
+//     rest in memory;
+//   - the at most POSLB_RUN cells inside the lane's run are loaded
+//     independently, and each entry takes the start of the first of them
+//     at or after its ID; runs with no cell in them (the usual case in
+//     sparse stretches) take the next cell's start without more loads.
 constexpr uint32_t POSLB_RUN = 16;
 __global__ __launch_bounds__(GB) void k_pos_lb(const uint64_t *cellLinear, const uint32_t *cellStart,
                                                uint64_t nne, uint64_t tc, uint32_t *posLB) {
@@ -289,21 +296,52 @@ __global__ __launch_bounds__(GB) void k_pos_lb(const uint64_t *cellLinear, const
             hi = next < hi ? next : hi;
         }
     }
+    // every lane active here: the shuffles read all 64 lanes
     const uint64_t L0 = t * POSLB_RUN;
-    if (L0 > tc) return;
-    // this lane's first cell: at most L0 - Lw cells past the wave's
-    uint64_t a = lo, b = lo + (L0 - Lw) < nne ? lo + (L0 - Lw) : nne;
-    while (a < b) {
-        const uint64_t mid = (a + b) >> 1;
-        if (cellLinear[mid] < L0) a = mid + 1; else b = mid;
+    const uint64_t cj = lo + lane < nne ? cellLinear[lo + lane] : ~0ull;
+    uint32_t c = 0;  // cells lo .. lo+63 below L0 (they are sorted)
+#pragma unroll
+    for (uint32_t step = 32; step; step >>= 1) {
+        const uint64_t v = __shfl(cj, (int)(c + step - 1), 64);
+        if (v < L0) c += step;
     }
+    if (__shfl(cj, 63, 64) < L0) c = 64;  // c was 63: the 64th is below too
+    uint64_t a = lo + c;  // first index with cellLinear >= L0 when c < 64
+    if (L0 > tc) return;
+    if (c == 64) {
+        // at most L0 - Lw cells lie in [Lw, L0): the first at or after L0 is
+        // within lo + (L0 - Lw)
+        uint64_t b = lo + (L0 - Lw) < nne ? lo + (L0 - Lw) : nne;
+        while (a < b) {
+            const uint64_t mid = (a + b) >> 1;
+            if (cellLinear[mid] < L0) a = mid + 1; else b = mid;
+        }
+    }
+    // a <= nne; cellStart holds nne + 1 entries
     const uint64_t L1 = L0 + POSLB_RUN < tc + 1 ? L0 + POSLB_RUN : tc + 1;
     uint32_t v[POSLB_RUN];
+    const uint64_t c0 = a < nne ? cellLinear[a] : ~0ull;
+    const uint32_t s0 = cellStart[a];
+    if (c0 >= L0 + POSLB_RUN) {
 #pragma unroll
-    for (uint32_t q = 0; q < POSLB_RUN; q++) {
-        const uint64_t L = L0 + q;
-        while (a < nne && cellLinear[a] < L) a++;
-        v[q] = cellStart[a];
+        for (uint32_t q = 0; q < POSLB_RUN; q++) v[q] = s0;
+    } else {
+        uint64_t cl[POSLB_RUN];
+        uint32_t cs[POSLB_RUN + 1];
+        cl[0] = c0;
+        cs[0] = s0;
+#pragma unroll
+        for (uint32_t q = 1; q < POSLB_RUN; q++) cl[q] = a + q < nne ? cellLinear[a + q] : ~0ull;
+#pragma unroll
+        for (uint32_t q = 1; q <= POSLB_RUN; q++) cs[q] = a + q <= nne ? cellStart[a + q] : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < POSLB_RUN; q++) {
+            uint32_t x = cs[0];
+#pragma unroll
+            for (uint32_t j = 0; j < POSLB_RUN; j++)
+                if (cl[j] < L0 + q) x = cs[j + 1];
+            v[q] = x;
+        }
     }
     if (L1 - L0 == POSLB_RUN) {
         uint4 *o = reinterpret_cast<uint4 *>(posLB + L0);
