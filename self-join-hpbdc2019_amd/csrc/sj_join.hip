@@ -1235,10 +1235,13 @@ __global__ __launch_bounds__(JB) void k_tile_heads(JoinArgs a, uint32_t q0, uint
     head[i] = ((s - rs) & 63u) == 0 ? 1u : 0u;
 }
 
+// zero (when set): the 2 * 256 work-queue class counters, cleared here
 __global__ __launch_bounds__(JB) void k_tile_compact(const uint32_t *head, const uint32_t *ex, uint32_t q0,
-                                                     uint32_t q1, uint32_t *tileStart, uint32_t *nTiles) {
+                                                     uint32_t q1, uint32_t *tileStart, uint32_t *nTiles,
+                                                     uint32_t *zero) {
     const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
     const uint64_t len = q1 - q0;
+    if (zero && i < 2 * 256) zero[i] = 0u;
     if (i >= len) return;
     if (head[i]) tileStart[ex[i]] = q0 + (uint32_t)i;
     if (i == len - 1) {
@@ -1314,17 +1317,23 @@ __device__ __forceinline__ uint32_t cost_key(uint32_t v) {
     return 255u - min(255u, 8u * (uint32_t)e + m);
 }
 
+// Work-queue classes: the 256 cost buckets.  bucket[0..QB) counts the tiles
+// of each class, bucket[QB..2 QB) are the placement cursors; both start at
+// zero (k_tile_compact clears them).
+constexpr uint32_t QB = 256;
+static_assert(JB == (int)QB, "one thread per class");
+
 // Tile cost: the candidates the tile stages, i.e. the union of its
-// neighbour cell ranges over the 3^(k-1) neighbour rows.  Entries from
-// nTiles to bound get the last key, so after a stable sort the first
-// nTiles entries are the tiles, heaviest bucket first and in cell order
-// inside a bucket (neighbouring tiles stay close in the queue).
+// neighbour cell ranges over the 3^(k-1) neighbour rows, as a class
+// (cost_key) per tile, counted per class in bucket[].
 template <int K>
-__global__ __launch_bounds__(JB) void k_tile_cost(JoinArgs a, uint32_t bound, uint32_t *key, uint32_t *val) {
+__global__ __launch_bounds__(JB) void k_tile_cost(JoinArgs a, uint32_t bound, uint32_t *key, uint32_t *bucket) {
+    __shared__ uint32_t hist[QB];
+    hist[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t t = blockIdx.x * JB + threadIdx.x;
-    if (t >= bound) return;
     uint32_t cost = 0;
-    if (t < *a.nTilesDev) {
+    if (t < bound && t < *a.nTilesDev) {
         const uint32_t s0 = a.tileStart[t], s1 = a.tileStart[t + 1] - 1u;
         const uint32_t cA = a.cellOf[s0], cB = a.cellOf[s1];
         uint32_t c[K];
@@ -1367,11 +1376,71 @@ __global__ __launch_bounds__(JB) void k_tile_cost(JoinArgs a, uint32_t bound, ui
                 }
             }
         }
-        key[t] = cost_key(cost);
-    } else {
-        key[t] = 255u;
+        const uint32_t kk = cost_key(cost);
+        key[t] = kk;
+        atomicAdd(&hist[kk], 1u);
     }
-    val[t] = t;
+    __syncthreads();
+    const uint32_t h = hist[threadIdx.x];
+    if (h) atomicAdd(&bucket[threadIdx.x], h);
+}
+
+// The queue: tiles by class, heaviest first.  Each block takes the class
+// offsets from the counts (a 256-entry scan of its own), ranks its tiles
+// stably inside each class (wave64 ballots over the 8 key bits, then the
+// waves in order) and reserves one run per class with one atomic, so a
+// class holds runs of tiles in cell order (neighbouring tiles stay close
+// in the queue).  Positions stay below nTiles: the runs of a class add up
+// to its count.
+__global__ __launch_bounds__(JB) void k_tile_place(const uint32_t *nTilesDev, const uint32_t *key, uint32_t *bucket,
+                                                   uint32_t *order) {
+    __shared__ uint32_t base[QB];                // the block's first position in each class
+    __shared__ uint32_t wcnt[JB / 64][QB];       // tiles of each class per wave, then offsets
+    __shared__ uint32_t wsum[JB / 64];
+    const unsigned lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t nT = *nTilesDev;
+    // class offsets: exclusive scan of the counts
+    const uint32_t x = bucket[threadIdx.x];
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(inc, off, 64);
+        if (lane >= (unsigned)off) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+#pragma unroll
+    for (int w = 0; w < JB / 64; w++) wcnt[w][threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t before = 0;
+    for (unsigned w = 0; w < wave; w++) before += wsum[w];
+    const uint32_t classBase = before + inc - x;
+    // stable rank of each tile among the wave's tiles of its class
+    const uint32_t t = blockIdx.x * JB + threadIdx.x;
+    const bool valid = t < nT;
+    const uint32_t kk = valid ? key[t] : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (kk >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    if (valid && rank == 0) wcnt[wave][kk] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+        // class threadIdx.x: the waves' offsets in order, then one run for the block
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < JB / 64; w++) {
+            const uint32_t c = wcnt[w][threadIdx.x];
+            wcnt[w][threadIdx.x] = run;
+            run += c;
+        }
+        base[threadIdx.x] = run ? classBase + atomicAdd(&bucket[QB + threadIdx.x], run) : 0u;
+    }
+    __syncthreads();
+    if (valid) order[base[kk] + wcnt[wave][kk] + rank] = t;
 }
 
 // Pieces a tile walk may execute per tile (tile_piece_bound); zero
@@ -1706,7 +1775,7 @@ static sj_status cell_workload(sj_grid_s *g, int lid, DBuf<uint32_t> &w) {
 
 // Row-aligned tiles over [q0, q1); the count stays on the device.
 sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
-                             DBuf<uint32_t> &nTiles) {
+                             DBuf<uint32_t> &nTiles, uint32_t *queueClasses) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t len = q1 - q0;
@@ -1722,8 +1791,8 @@ sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1,
     hipLaunchKernelGGL(k_tile_heads, dim3(nblk(len)), dim3(JB), 0, s, a, q0, q1, head.p);
     SJ_LAUNCH_CHECK();
     SJ_TRY(scan_exclusive_u32_to_u32(pool, s, head.p, len, ex.p));
-    hipLaunchKernelGGL(k_tile_compact, dim3(nblk(len)), dim3(JB), 0, s, head.p, ex.p, q0, q1, tileStart.p,
-                       nTiles.p);
+    hipLaunchKernelGGL(k_tile_compact, dim3(std::max<uint64_t>(nblk(len), nblk(2 * QB))), dim3(JB), 0, s, head.p,
+                       ex.p, q0, q1, tileStart.p, nTiles.p, queueClasses);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
@@ -1731,7 +1800,8 @@ sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1,
 // Work queue: tiles by descending staged-candidate bucket.  The tile count
 // stays on the device: the sort covers an upper bound (one tile per 64
 // points plus one per non-empty row, at most one per non-empty cell).
-static sj_status build_tile_queue(sj_grid_s *g, const JoinArgs &a, uint64_t len, DBuf<uint32_t> &order) {
+static sj_status build_tile_queue(sj_grid_s *g, const JoinArgs &a, uint64_t len, DBuf<uint32_t> &order,
+                                  uint32_t *classes) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t bound64 = (len + 63) / 64 + std::min<uint64_t>(len, g->nNonEmpty);
@@ -1740,17 +1810,14 @@ static sj_status build_tile_queue(sj_grid_s *g, const JoinArgs &a, uint64_t len,
         return SJ_ERR_OVERFLOW;
     }
     const uint32_t bound = (uint32_t)bound64;
-    DBuf<uint32_t> key, key2, val2;
+    DBuf<uint32_t> key;
     SJ_TRY(key.alloc(pool, bound));
-    SJ_TRY(key2.alloc(pool, bound));
     SJ_TRY(order.alloc(pool, bound));
-    SJ_TRY(val2.alloc(pool, bound));
     if (bound == 0) return SJ_OK;
-    hipLaunchKernelGGL(pick_tile_cost(g->k), dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, key.p, order.p);
+    hipLaunchKernelGGL(pick_tile_cost(g->k), dim3(nblk(bound)), dim3(JB), 0, s, a, bound, key.p, classes);
     SJ_LAUNCH_CHECK();
-    uint32_t *kp = key.p, *ka = key2.p, *vp = order.p, *va = val2.p;
-    SJ_TRY(radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, bound, 8));
-    if (vp != order.p) std::swap(order, val2);
+    hipLaunchKernelGGL(k_tile_place, dim3(nblk(bound)), dim3(JB), 0, s, a.nTilesDev, key.p, classes, order.p);
+    SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
 
@@ -2120,13 +2187,15 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     }
 
     // row-aligned tiles; work queue of tiles in descending workload
-    DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr;
-    SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev));
+    DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr, qclass;
+    const bool queue = P->schedule == SJ_SCHED_WORKQUEUE;
+    if (queue) SJ_TRY(qclass.alloc(pool, 2 * QB));
+    SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev, queue ? qclass.p : nullptr));
     a.tileStart = tileStart.p;
     a.nTilesDev = nTilesDev.p;
-    if (P->schedule == SJ_SCHED_WORKQUEUE) {
+    if (queue) {
         tw.start();
-        SJ_TRY(build_tile_queue(g, a, n, tileOrder));
+        SJ_TRY(build_tile_queue(g, a, n, tileOrder, qclass.p));
         tw.stop();
         SJ_TRY(qctr.alloc(pool, QSLICES * QSTRIDE));
         a.tileOrder = tileOrder.p;

@@ -543,8 +543,9 @@ __device__ __forceinline__ void emit_chunk32(const JoinArgs &a, uint32_t cb, uin
 // ------------------------------------------------- host helpers (sj_join.hip)
 double eps_threshold(double eps);
 JoinArgs base_args(sj_grid_s *g, int form, double T);
+// queueClasses (when set): the work queue's 2 * 256 class counters, cleared
 sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
-                      DBuf<uint32_t> &nTiles);
+                      DBuf<uint32_t> &nTiles, uint32_t *queueClasses = nullptr);
 sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec);
 
 // One persistent launch over (instance, tile) work items of many joins;
