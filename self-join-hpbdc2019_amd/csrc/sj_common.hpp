@@ -130,6 +130,10 @@ struct GridParams {
     uint64_t totalCells;
     uint32_t flags;          // bit0 non-finite coordinate, bit1 overflow
     uint32_t pad;
+    // written by the later build kernels: their error flags (GP_*) and the
+    // non-empty cell count, read back with the parameters in one copy
+    uint32_t err;
+    uint32_t nne;
 };
 enum { GP_NONFINITE = 1u, GP_OVERFLOW = 2u };
 

@@ -10,6 +10,7 @@
 //   6. coordinates gathered into cell order, per-cell coordinates and the
 //      O(1) dense lookup table when the grid is small enough.
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 
@@ -365,6 +366,12 @@ static int bits_for(uint64_t maxv) {
     return b;
 }
 
+// pinned words 16.. hold the grid parameters read back by the host
+static GridParams *pinned_params(sj_context ctx) {
+    static_assert(16 * 8 + sizeof(GridParams) <= 64 * 8, "pinned readback space");
+    return reinterpret_cast<GridParams *>(ctx->pinned + 16);
+}
+
 template <class KT>
 static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t *d_flags, int bits) {
     Pool *pool = &g->ctx->pool;
@@ -400,9 +407,10 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
     SJ_LAUNCH_CHECK();
     // sorted point IDs (indexLookupArr): the sort's value buffer itself
     g->sortedIds = std::move(vp == vals.p ? vals : vals2);
-    // the error flags and nNonEmpty (d_flags[1], written by the compaction),
-    // read back by grid_build once the gather behind them is enqueued
-    SJ_TRY_HIP(hipMemcpyAsync(g->ctx->pinned, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    // the parameters with the error flags and nNonEmpty (d_flags[1], written
+    // by the compaction) in one copy, read by grid_build once the gather
+    // behind them is enqueued
+    SJ_TRY_HIP(hipMemcpyAsync(pinned_params(g->ctx), g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
     SJ_TRY_HIP(hipEventRecord(g->ctx->ev, s));
     return SJ_OK;
 }
@@ -431,19 +439,16 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
 
     const unsigned nbmm = (unsigned)std::min<uint64_t>(1024, blocks_for(n));
     DBuf<double> partial;
-    DBuf<uint32_t> flags;
     SJ_TRY(partial.alloc(pool, (uint64_t)nbmm * (2 * k + 1)));
-    SJ_TRY(flags.alloc(pool, 2));  // error flags (set first by k_grid_params), then the non-empty cell count
     SJ_TRY(g->d_gp.alloc(pool, 1));
+    // error flags (set first by k_grid_params), then the non-empty cell count
+    uint32_t *flags = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(g->d_gp.p) + offsetof(GridParams, err));
+    static_assert(offsetof(GridParams, nne) == offsetof(GridParams, err) + 4, "flags, then the count");
     hipLaunchKernelGGL(pick_minmax(d), dim3(nbmm), dim3(GB), 0, s, d_points, n, d, k, partial.p);
     SJ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p,
-                       flags.p);
+    hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(GB), 0, s, partial.p, (int)nbmm, k, eps, g->d_gp.p, flags);
     SJ_LAUNCH_CHECK();
-    // the parameters land in pinned words 16.. (the flags use word 0)
-    GridParams *pgp = reinterpret_cast<GridParams *>(ctx->pinned + 16);
-    static_assert(16 * 8 + sizeof(GridParams) <= 64 * 8, "pinned readback space");
-    SJ_TRY_HIP(hipMemcpyAsync(pgp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
+    GridParams *pgp = pinned_params(ctx);
     auto check_params = [&]() -> sj_status {
         g->gp = *pgp;
         if (g->gp.flags & GP_NONFINITE) {
@@ -465,6 +470,7 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     int bits = h.bits;
     bool wide = h.wide;
     if (!guess) {
+        SJ_TRY_HIP(hipMemcpyAsync(pgp, g->d_gp.p, sizeof(GridParams), hipMemcpyDeviceToHost, s));
         SJ_TRY_HIP(hipStreamSynchronize(s));
         SJ_TRY(check_params());
         bits = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
@@ -472,9 +478,9 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
     }
     for (int attempt = 0;; attempt++) {
         if (wide)
-            SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags.p, bits));
+            SJ_TRY(sort_and_compact<uint64_t>(g, d_points, flags, bits));
         else
-            SJ_TRY(sort_and_compact<uint32_t>(g, d_points, flags.p, bits));
+            SJ_TRY(sort_and_compact<uint32_t>(g, d_points, flags, bits));
         // the coordinates in cell order need no host value: enqueued before
         // the host waits for the flags and the non-empty cell count
         SJ_TRY(g->sortedPts.alloc(pool, n * (uint64_t)d));
@@ -483,7 +489,7 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
         SJ_LAUNCH_CHECK();
         SJ_TRY_HIP(hipEventSynchronize(ctx->ev));
         if (guess && attempt == 0) {
-            SJ_TRY(check_params());  // copied before the event
+            SJ_TRY(check_params());  // copied with the flags before the event
             const int need = bits_for(g->gp.totalCells ? g->gp.totalCells - 1 : 0);
             const bool needWide = g->gp.totalCells > 0xFFFFFFFFull;
             if (need > bits || (needWide && !wide)) {
@@ -494,14 +500,12 @@ sj_status grid_build(sj_context ctx, const double *d_points, uint64_t n, int d, 
         }
         break;
     }
-    {
-        const uint64_t v = ctx->pinned[0];
-        if ((uint32_t)(v & 0xffffffffu) & GP_OVERFLOW) {
-            set_error("cell coordinate outside the grid (overflow)");
-            return SJ_ERR_OVERFLOW;
-        }
-        g->nNonEmpty = (uint32_t)(v >> 32);
+    g->gp = *pgp;  // the last copy: parameters, flags and the non-empty cell count
+    if (g->gp.err & GP_OVERFLOW) {
+        set_error("cell coordinate outside the grid (overflow)");
+        return SJ_ERR_OVERFLOW;
     }
+    g->nNonEmpty = g->gp.nne;
     ctx->gridHint.valid = true;
     ctx->gridHint.wide = g->gp.totalCells > 0xFFFFFFFFull;
     ctx->gridHint.d = d;
