@@ -299,10 +299,12 @@ sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint6
 }
 
 // -------------------------------------------------------------- radix sort
-// LSD, 8-bit digits.  Tile = 256 threads x 16 rounds.  Upsweep counts the
-// digits of every tile (hist[digit][tile]); an exclusive scan of that
-// digit-major table gives each (digit, tile) its output base; the
-// downsweep ranks each round of 256 keys stably with wave64 ballots
+// LSD, 8-bit digits, three launches per pass.  Tile = 256 threads x 16
+// rounds.  Upsweep counts the digits of every tile (hist[digit][tile]); one
+// block per digit scans its row over the tiles (histScan) and keeps the
+// digit's total; the downsweep takes each (digit, tile)'s output base as the
+// digit's base (a 256-entry scan of the totals, done by every block) plus
+// that prefix, ranks each round of 256 keys stably with wave64 ballots
 // (peer mask of equal digits, popcount below the lane) plus a prefix over
 // the four waves, places the tile's keys in LDS in digit order and writes
 // them out run by run.
@@ -326,6 +328,35 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const K *keys, uint64_t
     hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = bins[threadIdx.x];
 }
 
+// block d: exclusive scan of hist[d][0..ntiles) into histScan[d][..], the
+// row's total into totals[d]
+constexpr int RD_ITEMS = 8;
+__global__ __launch_bounds__(RS_BLOCK) void k_radix_digit_scan(const uint32_t *hist, uint32_t ntiles,
+                                                               uint32_t *histScan, uint32_t *totals) {
+    const uint64_t row = (uint64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < ntiles; c0 += RS_BLOCK * RD_ITEMS) {
+        // thread t owns entries [c0 + t * ITEMS, + ITEMS)
+        uint32_t v[RD_ITEMS], local = 0;
+#pragma unroll
+        for (int j = 0; j < RD_ITEMS; j++) {
+            const uint32_t i = c0 + threadIdx.x * RD_ITEMS + j;
+            v[j] = i < ntiles ? hist[row + i] : 0u;
+            local += v[j];
+        }
+        uint32_t tot;
+        uint32_t ex = carry + block_exclusive_scan<uint32_t>(local, &tot);
+#pragma unroll
+        for (int j = 0; j < RD_ITEMS; j++) {
+            const uint32_t i = c0 + threadIdx.x * RD_ITEMS + j;
+            if (i < ntiles) histScan[row + i] = ex;
+            ex += v[j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 __device__ __forceinline__ uint32_t popc_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -334,7 +365,7 @@ template <class K>
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K *kin, const uint32_t *vin, K *kout,
                                                             uint32_t *vout, uint64_t n, int shift,
                                                             const uint32_t *histScan, const uint32_t *hist,
-                                                            uint32_t ntiles) {
+                                                            const uint32_t *totals, uint32_t ntiles) {
     __shared__ uint32_t base[256];   // next output position of each digit
     __shared__ uint32_t gbase[256];  // the tile's first output position of each digit
     __shared__ uint32_t lbase[256];  // the tile's first slot of each digit in digit order
@@ -342,7 +373,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K *kin, const 
     __shared__ K sk[RS_TILE];
     __shared__ uint32_t sv[RS_TILE];
     const unsigned wave = threadIdx.x >> 6;
-    const uint32_t gb = histScan[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    uint32_t allKeys;
+    const uint32_t digitBase = block_exclusive_scan<uint32_t>(totals[threadIdx.x], &allKeys);
+    const uint32_t gb = digitBase + histScan[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
     base[threadIdx.x] = gb;
     gbase[threadIdx.x] = gb;
     uint32_t tileCount;
@@ -411,14 +444,16 @@ static sj_status radix_sort_pairs(Pool *pool, hipStream_t s, K **keys, K **keys_
     const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     DBuf<uint32_t> hist, histScan;
     SJ_TRY(hist.alloc(pool, (uint64_t)256 * ntiles));
-    SJ_TRY(histScan.alloc(pool, (uint64_t)256 * ntiles + 1));
+    SJ_TRY(histScan.alloc(pool, (uint64_t)256 * ntiles + 256));  // per-digit prefixes, then the totals
+    uint32_t *totals = histScan.p + (uint64_t)256 * ntiles;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL((k_radix_hist<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, n, shift, hist.p,
                            ntiles);
         SJ_LAUNCH_CHECK();
-        SJ_TRY(scan_exclusive_u32_to_u32(pool, s, hist.p, (uint64_t)256 * ntiles, histScan.p));
+        hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(RS_BLOCK), 0, s, hist.p, ntiles, histScan.p, totals);
+        SJ_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_radix_scatter<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, *vals,
-                           *keys_alt, *vals_alt, n, shift, histScan.p, hist.p, ntiles);
+                           *keys_alt, *vals_alt, n, shift, histScan.p, hist.p, totals, ntiles);
         SJ_LAUNCH_CHECK();
         std::swap(*keys, *keys_alt);
         std::swap(*vals, *vals_alt);
