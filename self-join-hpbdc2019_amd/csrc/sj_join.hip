@@ -1235,13 +1235,16 @@ __global__ __launch_bounds__(JB) void k_tile_heads(JoinArgs a, uint32_t q0, uint
     head[i] = ((s - rs) & 63u) == 0 ? 1u : 0u;
 }
 
-// zero (when set): the 2 * 256 work-queue class counters, cleared here
+// z: counters of the join that follows, cleared here (each when set)
 __global__ __launch_bounds__(JB) void k_tile_compact(const uint32_t *head, const uint32_t *ex, uint32_t q0,
                                                      uint32_t q1, uint32_t *tileStart, uint32_t *nTiles,
-                                                     uint32_t *zero) {
+                                                     TileZero z) {
     const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
     const uint64_t len = q1 - q0;
-    if (zero && i < 2 * 256) zero[i] = 0u;
+    if (z.classes && i < 2 * 256) z.classes[i] = 0u;
+    if (z.mir && i < z.n) z.mir[i] = 0u;
+    if (z.stat && i < 8) z.stat[i] = 0ull;
+    if (z.qctr && i < (uint64_t)QSLICES * QSTRIDE) z.qctr[i] = 0u;
     if (i >= len) return;
     if (head[i]) tileStart[ex[i]] = q0 + (uint32_t)i;
     if (i == len - 1) {
@@ -1446,8 +1449,11 @@ __global__ __launch_bounds__(JB) void k_tile_place(const uint32_t *nTilesDev, co
 // Pieces a tile walk may execute per tile (tile_piece_bound); zero
 // for entries from nTiles to bound.  Scanned, they lay out the hit record.
 template <int K>
-__global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, uint32_t cw, uint32_t *out) {
+// zeroCtr (when set): the hit-record counters, cleared here
+__global__ __launch_bounds__(JB) void k_tile_bound(JoinArgs a, uint32_t bound, uint32_t cw, uint32_t *out,
+                                                   unsigned long long *zeroCtr) {
     const uint32_t t = blockIdx.x * JB + threadIdx.x;
+    if (zeroCtr && t < REC_NCTR) zeroCtr[t] = 0ull;
     if (t >= bound) return;
     uint32_t v = 0;
     if (t < *a.nTilesDev) {
@@ -1486,19 +1492,6 @@ __device__ __forceinline__ void gather_count_scalars(const CountScalars &c, unsi
 }
 
 __global__ void k_count_scalars(CountScalars c) { gather_count_scalars(c, threadIdx.x); }
-
-// The count pass's counters at zero: mirrored-hit counts of the n rows,
-// statistics, and (when not null) the queue slice counters and the
-// hit-record counters.
-__global__ __launch_bounds__(JB) void k_zero_count_state(uint32_t *mir, uint64_t n, unsigned long long *stat,
-                                                         uint32_t *qctr, unsigned long long *recCtr) {
-    const uint64_t i = (uint64_t)blockIdx.x * JB + threadIdx.x;
-    if (i < n) mir[i] = 0u;
-    if (i < 8) stat[i] = 0ull;
-    if (qctr && i < (uint64_t)QSLICES * QSTRIDE) qctr[i] = 0u;
-    if (recCtr && i < REC_NCTR) recCtr[i] = 0ull;
-}
-
 
 // mirrored-hit cursors of rows [w0, w1): start right after the own hits.
 // recBad (when set): only when the hit record is unusable (the fill then
@@ -1664,7 +1657,7 @@ static TileCostKernel pick_tile_cost(int k) {
     return nullptr;
 }
 
-using TileBoundKernel = void (*)(JoinArgs, uint32_t, uint32_t, uint32_t *);
+using TileBoundKernel = void (*)(JoinArgs, uint32_t, uint32_t, uint32_t *, unsigned long long *);
 static TileBoundKernel pick_tile_bound(int k) {
     switch (k) {
     case 1: return k_tile_bound<1>;
@@ -1775,7 +1768,7 @@ static sj_status cell_workload(sj_grid_s *g, int lid, DBuf<uint32_t> &w) {
 
 // Row-aligned tiles over [q0, q1); the count stays on the device.
 sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
-                             DBuf<uint32_t> &nTiles, uint32_t *queueClasses) {
+                             DBuf<uint32_t> &nTiles, const TileZero &zero) {
     Pool *pool = &g->ctx->pool;
     hipStream_t s = g->ctx->stream;
     const uint64_t len = q1 - q0;
@@ -1791,8 +1784,9 @@ sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1,
     hipLaunchKernelGGL(k_tile_heads, dim3(nblk(len)), dim3(JB), 0, s, a, q0, q1, head.p);
     SJ_LAUNCH_CHECK();
     SJ_TRY(scan_exclusive_u32_to_u32(pool, s, head.p, len, ex.p));
-    hipLaunchKernelGGL(k_tile_compact, dim3(std::max<uint64_t>(nblk(len), nblk(2 * QB))), dim3(JB), 0, s, head.p,
-                       ex.p, q0, q1, tileStart.p, nTiles.p, queueClasses);
+    const uint64_t nz = std::max<uint64_t>(len, std::max<uint64_t>(2 * QB, std::max<uint64_t>(zero.n, 256)));
+    hipLaunchKernelGGL(k_tile_compact, dim3(nblk(nz)), dim3(JB), 0, s, head.p, ex.p, q0, q1, tileStart.p, nTiles.p,
+                       zero);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
@@ -1951,7 +1945,7 @@ sj_status launch_tile_bound(const JoinArgs &a, int k, uint32_t bound, uint32_t c
         return SJ_ERR_UNSUPPORTED;
     }
     if (bound == 0) return SJ_OK;
-    hipLaunchKernelGGL(kfn, dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, cw, out);
+    hipLaunchKernelGGL(kfn, dim3((bound + JB - 1) / JB), dim3(JB), 0, s, a, bound, cw, out, nullptr);
     SJ_LAUNCH_CHECK();
     return SJ_OK;
 }
@@ -2187,28 +2181,33 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     }
 
     // row-aligned tiles; work queue of tiles in descending workload
-    DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr, qclass;
-    const bool queue = P->schedule == SJ_SCHED_WORKQUEUE;
-    if (queue) SJ_TRY(qclass.alloc(pool, 2 * QB));
-    SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev, queue ? qclass.p : nullptr));
-    a.tileStart = tileStart.p;
-    a.nTilesDev = nTilesDev.p;
-    if (queue) {
-        tw.start();
-        SJ_TRY(build_tile_queue(g, a, n, tileOrder, qclass.p));
-        tw.stop();
-        SJ_TRY(qctr.alloc(pool, QSLICES * QSTRIDE));
-        a.tileOrder = tileOrder.p;
-        a.qctr = qctr.p;
-    }
-
-    // count pass
+    // the count pass's counters (own and mirrored hits per row, statistics,
+    // queue counters), cleared by the tile compaction
     DBuf<uint32_t> own, mir;
     DBuf<uint64_t> scal;  // the host's scalars, the live statistics at SC_STAT
     SJ_TRY(own.alloc(pool, n));
     SJ_TRY(mir.alloc(pool, n));
     SJ_TRY(scal.alloc(pool, SC_WORDS));
     unsigned long long *const stat = reinterpret_cast<unsigned long long *>(scal.p) + SC_STAT;
+    DBuf<uint32_t> tileStart, nTilesDev, tileOrder, qctr, qclass;
+    const bool queue = P->schedule == SJ_SCHED_WORKQUEUE;
+    if (queue) {
+        SJ_TRY(qclass.alloc(pool, 2 * QB));
+        SJ_TRY(qctr.alloc(pool, QSLICES * QSTRIDE));
+    }
+    const TileZero tz{queue ? qclass.p : nullptr, mir.p, n, stat, queue ? qctr.p : nullptr};
+    SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev, tz));
+    a.tileStart = tileStart.p;
+    a.nTilesDev = nTilesDev.p;
+    if (queue) {
+        tw.start();
+        SJ_TRY(build_tile_queue(g, a, n, tileOrder, qclass.p));
+        tw.stop();
+        a.tileOrder = tileOrder.p;
+        a.qctr = qctr.p;
+    }
+
+    // count pass
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
     a.stat = stat;
@@ -2229,8 +2228,9 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             (void)hipGetLastError();
         } else {
             const uint32_t cw = a.mDP > 0 ? 32u : 64u;  // candidates per recorded piece
-            hipLaunchKernelGGL(pick_tile_bound(g->k), dim3(nblk(tBound)), dim3(JB), 0, s, a, (uint32_t)tBound, cw,
-                               tileBound.p);
+            // the record's counters cleared by the same launch
+            hipLaunchKernelGGL(pick_tile_bound(g->k), dim3(nblk(std::max<uint64_t>(tBound, REC_NCTR))), dim3(JB), 0,
+                               s, a, (uint32_t)tBound, cw, tileBound.p, recCtr.p);
             SJ_LAUNCH_CHECK();
             SJ_TRY(scan_exclusive_u32_to_u64(pool, s, tileBound.p, tBound, tileDir.p));
             a.recDir = recDir.p;
@@ -2241,10 +2241,6 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             a.maskCap = maskCap;
         }
     }
-    const uint64_t nz = std::max<uint64_t>(n, std::max<uint64_t>(REC_NCTR, QSLICES * QSTRIDE));
-    hipLaunchKernelGGL(k_zero_count_state, dim3(nblk(nz)), dim3(JB), 0, s, mir.p, n, stat,
-                       a.tileOrder ? qctr.p : nullptr, recording ? recCtr.p : nullptr);
-    SJ_LAUNCH_CHECK();
     tc.start();
     SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();

@@ -543,9 +543,18 @@ __device__ __forceinline__ void emit_chunk32(const JoinArgs &a, uint32_t cb, uin
 // ------------------------------------------------- host helpers (sj_join.hip)
 double eps_threshold(double eps);
 JoinArgs base_args(sj_grid_s *g, int form, double T);
-// queueClasses (when set): the work queue's 2 * 256 class counters, cleared
+// Counters the tile compaction clears for the join that follows (each when
+// set): the work queue's 2 * 256 class counters, n mirrored-hit counts, 8
+// statistics, the queue slice counters.
+struct TileZero {
+    uint32_t *classes = nullptr;
+    uint32_t *mir = nullptr;
+    uint64_t n = 0;
+    unsigned long long *stat = nullptr;
+    uint32_t *qctr = nullptr;
+};
 sj_status build_tiles(sj_grid_s *g, const JoinArgs &a, uint32_t q0, uint32_t q1, DBuf<uint32_t> &tileStart,
-                      DBuf<uint32_t> &nTiles, uint32_t *queueClasses = nullptr);
+                      DBuf<uint32_t> &nTiles, const TileZero &zero = TileZero{});
 sj_status prepare_fp32(sj_grid_s *g, JoinArgs &a, int *prec);
 
 // One persistent launch over (instance, tile) work items of many joins;
