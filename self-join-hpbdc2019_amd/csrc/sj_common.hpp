@@ -159,11 +159,17 @@ sj_status scan_exclusive_heads(Pool *pool, hipStream_t s, const uint64_t *keys, 
 sj_status scan_exclusive_sum2_u32_to_u64(Pool *pool, hipStream_t s, const uint32_t *a, const uint32_t *b,
                                          uint64_t n, uint64_t *out /* n+1 */);
 // Stable LSD radix sort of (key,val) by the low `bits` bits of key.  The
-// sorted data ends in (*keys,*vals); the other buffers are scratch.
+// sorted data ends in (*keys,*vals); the other buffers are scratch.  hist0
+// (when set): the caller's counts of the lowest 8-bit digit per sort tile of
+// RADIX_TILE keys, digit-major (hist0[digit * radix_tiles(n) + tile]).
+constexpr uint32_t RADIX_TILE = 4096;
+uint32_t radix_tiles(uint64_t n);
 sj_status radix_sort_pairs_u64(Pool *pool, hipStream_t s, uint64_t **keys, uint64_t **keys_alt,
-                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits);
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits,
+                               const uint32_t *hist0 = nullptr);
 sj_status radix_sort_pairs_u32(Pool *pool, hipStream_t s, uint32_t **keys, uint32_t **keys_alt,
-                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits);
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits,
+                               const uint32_t *hist0 = nullptr);
 sj_status fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 sj_status fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 

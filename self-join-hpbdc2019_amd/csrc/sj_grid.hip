@@ -195,28 +195,45 @@ __global__ __launch_bounds__(GB) void k_grid_params(const double *partial, int n
 }
 
 // Cell coordinate (unsigned)((x - minArr)/eps): FP64 subtract, correctly
-// rounded divide, truncation (main.cu:263,294; kernel.cu:1457).
+// rounded divide, truncation (main.cu:263,294; kernel.cu:1457).  One block
+// per sort tile of RADIX_TILE points: the keys and values, and the tile's
+// count of each lowest 8-bit key digit for the sort's first pass
+// (hist0[digit * ntiles + tile]).
+constexpr int KB = 1024;  // threads of the key kernel: four keys each per sort tile
+static_assert(RADIX_TILE % KB == 0 && KB >= 256, "sort tile");
 template <class KT>
-__global__ __launch_bounds__(GB) void k_cell_keys(const double *pts, uint64_t n, int d, int k,
+__global__ __launch_bounds__(KB) void k_cell_keys(const double *pts, uint64_t n, int d, int k,
                                                   double eps, const GridParams *gp, KT *keys,
-                                                  uint32_t *vals, uint32_t *flags) {
-    uint64_t i = (uint64_t)blockIdx.x * GB + threadIdx.x;
-    if (i >= n) return;
-    const double *p = pts + i * (uint64_t)d;
-    uint64_t lin = 0;
+                                                  uint32_t *vals, uint32_t *flags, uint32_t *hist0,
+                                                  uint32_t ntiles) {
+    __shared__ uint32_t bins[256];
+    if (threadIdx.x < 256) bins[threadIdx.x] = 0u;
+    __syncthreads();
     bool bad = false;
-    for (int j = 0; j < k; j++) {
-        double v = (p[j] - gp->minArr[j]) / eps;
-        uint32_t c = 0;
-        if (!(v >= 0.0) || !(v < (double)gp->nCells[j]))
-            bad = true;
-        else
-            c = (uint32_t)v;
-        lin += (uint64_t)c * gp->mult[j];
+    const uint64_t base = (uint64_t)blockIdx.x * RADIX_TILE;
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_TILE / KB; r++) {
+        const uint64_t i = base + (uint64_t)r * KB + threadIdx.x;
+        if (i < n) {
+            const double *p = pts + i * (uint64_t)d;
+            uint64_t lin = 0;
+            for (int j = 0; j < k; j++) {
+                double v = (p[j] - gp->minArr[j]) / eps;
+                uint32_t c = 0;
+                if (!(v >= 0.0) || !(v < (double)gp->nCells[j]))
+                    bad = true;
+                else
+                    c = (uint32_t)v;
+                lin += (uint64_t)c * gp->mult[j];
+            }
+            keys[i] = (KT)lin;
+            vals[i] = (uint32_t)i;
+            atomicAdd(&bins[(uint32_t)lin & 255u], 1u);
+        }
     }
     if (bad) atomicOr(flags, (uint32_t)GP_OVERFLOW);
-    keys[i] = (KT)lin;
-    vals[i] = (uint32_t)i;
+    __syncthreads();
+    if (threadIdx.x < 256) hist0[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = bins[threadIdx.x];
 }
 
 template <class KT>
@@ -383,16 +400,19 @@ static sj_status sort_and_compact(sj_grid_s *g, const double *d_points, uint32_t
     SJ_TRY(keys2.alloc(pool, n));
     SJ_TRY(vals.alloc(pool, n));
     SJ_TRY(vals2.alloc(pool, n));
-    hipLaunchKernelGGL((k_cell_keys<KT>), dim3(blocks_for(n)), dim3(GB), 0, s, d_points, n, g->d, g->k,
-                       g->eps, g->d_gp.p, keys.p, vals.p, d_flags);
+    const uint32_t nt = radix_tiles(n);
+    DBuf<uint32_t> hist0;  // the sort's first histogram, counted with the keys
+    SJ_TRY(hist0.alloc(pool, (uint64_t)256 * nt));
+    hipLaunchKernelGGL((k_cell_keys<KT>), dim3(nt), dim3(KB), 0, s, d_points, n, g->d, g->k, g->eps, g->d_gp.p,
+                       keys.p, vals.p, d_flags, hist0.p, nt);
     SJ_LAUNCH_CHECK();
     KT *kp = keys.p, *ka = keys2.p;
     uint32_t *vp = vals.p, *va = vals2.p;
     sj_status st;
     if constexpr (sizeof(KT) == 8)
-        st = radix_sort_pairs_u64(pool, s, &kp, &ka, &vp, &va, n, bits);
+        st = radix_sort_pairs_u64(pool, s, &kp, &ka, &vp, &va, n, bits, hist0.p);
     else
-        st = radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, n, bits);
+        st = radix_sort_pairs_u32(pool, s, &kp, &ka, &vp, &va, n, bits, hist0.p);
     SJ_TRY(st);
 
     // cell index of each sorted position: the scan of the run heads

@@ -2360,15 +2360,13 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             hipLaunchKernelGGL(k_mir_cursors, dim3(nbq), dim3(JB), 0, s, r->d_offsets.p, own.p, 0u,
                                (uint32_t)n, 0ull, mirCur.p, a.mDP == 0 ? recCtr.p + REC_BAD : nullptr, qz, gsc);
             SJ_LAUNCH_CHECK();
+            // replay when the record is usable; an unusable one (the
+            // record's space fell short, seen on the host afterwards) is
+            // evaluated in a second round
             JoinArgs ar = a;
             ar.recGate = recCtr.p + REC_BAD;
-            ar.recGateMode = 1;  // replay: the record is usable
+            ar.recGateMode = 1;
             SJ_TRY(launch_join(g, ar, PASS_FILL, prec, true));
-            // evaluate: the record is not.  Exactly one of the two runs, so
-            // the queue counters the cursor kernel zeroed are still zero
-            // when this one does
-            ar.recGateMode = 2;
-            SJ_TRY(launch_join(g, ar, PASS_FILL, prec, false));
         }
         return SJ_OK;
     };
@@ -2405,11 +2403,17 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             SJ_TRY_HIP(hipStreamSynchronize(s));
             if (known) break;
             SJ_TRY(absorb(pin));
-            if (!pin[SC_STAT + STAT_OVF]) break;
-            // the result outgrew the buffer: fill again into one of its size
+            // the gated replay skips an unusable record before its capacity
+            // check, so the size is checked here too
+            const bool outgrew = pin[SC_STAT + STAT_OVF] != 0 || r->total > cap;
+            if (!outgrew && (replay || !recording)) break;
+            // fill again: into a buffer of the result's size when it outgrew
+            // this one, evaluating distances when the record was unusable
             known = true;
-            r->stats.overflow_retries = 1;
-            SJ_TRY(r->d_ids.alloc(pool, r->total));
+            if (outgrew) {
+                r->stats.overflow_retries = 1;
+                SJ_TRY(r->d_ids.alloc(pool, r->total));
+            }
             SJ_TRY(fill_u64(s, reinterpret_cast<uint64_t *>(stat) + STAT_ERR, 1, 0ull));
         }
         if (pin[SC_STAT + STAT_ERR]) {

@@ -311,6 +311,7 @@ sj_status scan_exclusive_u32_to_u64_tmp(hipStream_t s, const uint32_t *in, uint6
 constexpr int RS_BLOCK = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr uint64_t RS_TILE = (uint64_t)RS_BLOCK * RS_ROUNDS;
+static_assert(RS_TILE == RADIX_TILE, "sort tile");
 
 template <class K>
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const K *keys, uint64_t n, int shift,
@@ -433,9 +434,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(const K *kin, const 
     }
 }
 
+uint32_t radix_tiles(uint64_t n) { return (uint32_t)((n + RS_TILE - 1) / RS_TILE); }
+
 template <class K>
 static sj_status radix_sort_pairs(Pool *pool, hipStream_t s, K **keys, K **keys_alt, uint32_t **vals,
-                                  uint32_t **vals_alt, uint64_t n, int bits) {
+                                  uint32_t **vals_alt, uint64_t n, int bits, const uint32_t *hist0) {
     if (n <= 1 || bits <= 0) return SJ_OK;
     if (n > 0xFFFFFFFFull) {
         set_error("radix sort supports at most 2^32-1 keys");
@@ -447,13 +450,17 @@ static sj_status radix_sort_pairs(Pool *pool, hipStream_t s, K **keys, K **keys_
     SJ_TRY(histScan.alloc(pool, (uint64_t)256 * ntiles + 256));  // per-digit prefixes, then the totals
     uint32_t *totals = histScan.p + (uint64_t)256 * ntiles;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL((k_radix_hist<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, n, shift, hist.p,
-                           ntiles);
-        SJ_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(RS_BLOCK), 0, s, hist.p, ntiles, histScan.p, totals);
+        // the lowest digit's histogram may come with the keys
+        const uint32_t *h = shift == 0 && hist0 ? hist0 : hist.p;
+        if (h == hist.p) {
+            hipLaunchKernelGGL((k_radix_hist<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, n, shift, hist.p,
+                               ntiles);
+            SJ_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(RS_BLOCK), 0, s, h, ntiles, histScan.p, totals);
         SJ_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_radix_scatter<K>), dim3(ntiles), dim3(RS_BLOCK), 0, s, *keys, *vals,
-                           *keys_alt, *vals_alt, n, shift, histScan.p, hist.p, totals, ntiles);
+                           *keys_alt, *vals_alt, n, shift, histScan.p, h, totals, ntiles);
         SJ_LAUNCH_CHECK();
         std::swap(*keys, *keys_alt);
         std::swap(*vals, *vals_alt);
@@ -462,13 +469,13 @@ static sj_status radix_sort_pairs(Pool *pool, hipStream_t s, K **keys, K **keys_
 }
 
 sj_status radix_sort_pairs_u64(Pool *pool, hipStream_t s, uint64_t **keys, uint64_t **keys_alt,
-                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits) {
-    return radix_sort_pairs<uint64_t>(pool, s, keys, keys_alt, vals, vals_alt, n, bits);
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits, const uint32_t *hist0) {
+    return radix_sort_pairs<uint64_t>(pool, s, keys, keys_alt, vals, vals_alt, n, bits, hist0);
 }
 
 sj_status radix_sort_pairs_u32(Pool *pool, hipStream_t s, uint32_t **keys, uint32_t **keys_alt,
-                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits) {
-    return radix_sort_pairs<uint32_t>(pool, s, keys, keys_alt, vals, vals_alt, n, bits);
+                               uint32_t **vals, uint32_t **vals_alt, uint64_t n, int bits, const uint32_t *hist0) {
+    return radix_sort_pairs<uint32_t>(pool, s, keys, keys_alt, vals, vals_alt, n, bits, hist0);
 }
 
 }  // namespace sj

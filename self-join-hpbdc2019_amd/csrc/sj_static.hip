@@ -40,7 +40,7 @@ bool g_done[MAX_DEVICES] = {};
 constexpr uint64_t PAD = 256 * 64;  // alignment of up to 64 blocks
 
 uint64_t scan_state(uint64_t n) { return n / 8 + (1u << 20); }  // look-back scan state
-uint64_t radix_hist(uint64_t n) { return n / 2 + 4096; }        // histograms of one sort
+uint64_t radix_hist(uint64_t n) { return 3 * (n / 4) + 8192; }  // histograms of one sort (+ the first, given)
 
 uint64_t grid_need(uint64_t n, int d, int k) {
     uint64_t b = 1024 * (2 * 12 + 1) * 8 + 1024;       // min/max partials, flags, grid params

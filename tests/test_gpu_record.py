@@ -97,6 +97,35 @@ def test_record_overflow_then_replay():
         c.close()
 
 
+def test_unusable_record_in_ahead_fill():
+    """A context that has seen a larger result fills ahead of the host; when
+    the record then outgrows its space (6-D uniform, 300k points), the
+    replay is skipped on the device and the host evaluates the fill in a
+    second round, into the same buffer"""
+    import sjamd
+
+    pts = D.uniform(300_000, 6, 17)
+    eps = 0.2
+    _say("[record] ahead fill, unusable record, uniform6d_300k")
+    ref = O.join(pts, 6, eps, O.LID)
+    c = sjamd.Context(0)
+    try:
+        ra = c.self_join(D.uniform(60_000, 2, 23), 2, 0.08, form="lid")
+        big = ra.total
+        ra.free()
+        assert big > int(ref[0][-1]), "the earlier result must be the larger"
+        g = c.grid_build(pts, 6, eps)
+        r = g.join(form="lid")
+        st = r.stats()
+        _same(r.csr(), ref, "ahead, unusable record")
+        r.free()
+        g.free()
+        assert st["fill_from_record"] == 0, "the record was expected to outgrow the initial space"
+        assert st["overflow_retries"] == 0
+    finally:
+        c.close()
+
+
 def test_record_with_batches(ctx):
     """result rounds: every round's rows equal the oracle's"""
     pts, k, eps, ref = _case("uniform6d_50k")
