@@ -1718,6 +1718,12 @@ struct EvTimer {
         used = true;
     }
     void stop() { (void)hipEventRecord(b, s); }
+    // start where prev stopped, with no GPU work between: its end event is
+    // this one's start (one marker fewer in the stream).  Events not owned.
+    void start_at(const EvTimer &prev) {
+        a = prev.b;
+        used = true;
+    }
     double ms() {
         if (!used) return 0.0;
         float f = 0;
@@ -2246,7 +2252,7 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     tc.stop();
 
     // row offsets in cell order (the caller's array when it gave one)
-    ts.start();
+    ts.start_at(tc);  // nothing runs between the count and the scan
     if (r->extOffsets) {
         r->d_offsets.p = r->extOffsets;
         r->d_offsets.n = n + 1;
@@ -2390,7 +2396,14 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             SJ_TRY(r->d_ids.alloc(pool, cap));
         }
         for (int round = 0; round < 2; round++) {
-            tf.start();
+            // from the first round's start to the last round's end; ahead,
+            // nothing runs between the scan and the fill
+            if (round == 0) {
+                if (ahead)
+                    tf.start_at(ts);
+                else
+                    tf.start();
+            }
             SJ_TRY(launch_fill(known ? 0 : cap, known, round == 0 && ahead));
             tf.stop();
             if (P->sort_rows) {
