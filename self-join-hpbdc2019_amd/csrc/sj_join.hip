@@ -2205,15 +2205,8 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     SJ_TRY(build_tiles(g, a, 0u, (uint32_t)n, tileStart, nTilesDev, tz));
     a.tileStart = tileStart.p;
     a.nTilesDev = nTilesDev.p;
-    if (queue) {
-        tw.start();
-        SJ_TRY(build_tile_queue(g, a, n, tileOrder, qclass.p));
-        tw.stop();
-        a.tileOrder = tileOrder.p;
-        a.qctr = qctr.p;
-    }
-
-    // count pass
+    // count pass (the hit record's directory, then the work queue, then
+    // the count)
     a.ownCnt = own.p;
     a.mirCnt = mir.p;
     a.stat = stat;
@@ -2247,7 +2240,16 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
             a.maskCap = maskCap;
         }
     }
-    tc.start();
+    if (queue) {
+        tw.start();
+        SJ_TRY(build_tile_queue(g, a, n, tileOrder, qclass.p));
+        tw.stop();
+        a.tileOrder = tileOrder.p;
+        a.qctr = qctr.p;
+        tc.start_at(tw);  // nothing runs between the queue and the count
+    } else {
+        tc.start();
+    }
     SJ_TRY(launch_join(g, a, PASS_COUNT, prec));
     tc.stop();
 
