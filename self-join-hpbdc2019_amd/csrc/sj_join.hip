@@ -2340,8 +2340,10 @@ sj_status join(sj_grid_s *g, const sj_join_params *P, sj_result_s *r) {
     // the fill of the whole result into ids[cap]: the replayed LDS tile walk
     // places mirrored hits at the recorded reservations, every other fill
     // takes them from the cursors.  known: the record's state is known on the
-    // host (replay); otherwise both fills are launched, each gated on it.
-    // gather: the cursor kernel also gathers the count pass's scalars
+    // host (replay); otherwise, when recording, only the replay is launched,
+    // gated on the record being usable (the caller evaluates in a second
+    // round when it was not).  gather: the cursor kernel also gathers the
+    // count pass's scalars
     auto launch_fill = [&](uint64_t cap, bool known, bool gather) -> sj_status {
         a.ids = r->d_ids.p;
         a.outBase = 0;
